@@ -1,0 +1,111 @@
+/* libslabhip: C ABI of the MI355X (gfx950) SlabLU factor+solve kernels.
+ *
+ * Conventions
+ *   - dtype: SLAB_F64 (0) or SLAB_F32 (1); every pointer is device memory
+ *     owned by the caller (the library never frees caller memory).
+ *   - Matrices are strided views: element (i, j) of batch b is at
+ *       ptr + b*bs + i*rs + j*cs   (strides in elements).
+ *     Transposed operands are passed by swapping rs and cs.
+ *   - Pivots are int32, 1-based (LAPACK getrf convention).
+ *   - Every call is asynchronous on `stream` (a hipStream_t; NULL = the
+ *     default stream).  Return value: 0 on success, < 0 for an invalid
+ *     argument (-1) or a HIP launch error (-(1000 + hipError_t)).  Exactly
+ *     zero pivots are reported through the optional `info` device array
+ *     (1-based row of the first zero pivot, 0 if none), as LAPACK does.
+ *
+ * The Python host layer (slablu_amd/) keeps the reference's API and calls
+ * these entry points through ctypes; each declaration names the reference
+ * code it replaces (paths relative to the annayesy/slabLU checkout).
+ */
+#ifndef SLAB_HIP_H
+#define SLAB_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SLAB_F64 = 0, SLAB_F32 = 1 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int slab_version(void);
+
+/* Batched C = alpha*A@B + beta*C (A m x k, B k x n).
+ * Replaces the batched matmuls of src_disc/hps_parallel_leaf_ops.py:58 and
+ * src_solver/block_tridiag_solver_dense.py:169,193,200. */
+int slab_gemm(int dtype, int batch, int m, int n, int k, double alpha,
+              const void* A, int64_t a_rs, int64_t a_cs, int64_t a_bs,
+              const void* B, int64_t b_rs, int64_t b_cs, int64_t b_bs,
+              double beta, void* C, int64_t c_rs, int64_t c_cs, int64_t c_bs, void* stream);
+
+/* Batched left triangular solve B <- A^{-1} B, A m x m lower/upper (as
+ * viewed), unit or non-unit diagonal.  Replaces torch.linalg.solve_triangular
+ * in src_solver/block_tridiag_solver_dense.py:111-140. */
+int slab_trsm(int dtype, int batch, int m, int n, int lower, int unit,
+              const void* A, int64_t a_rs, int64_t a_cs, int64_t a_bs,
+              void* B, int64_t b_rs, int64_t b_cs, int64_t b_bs, void* stream);
+
+/* Batched row interchanges rows k1..k2-1 (forward or backward order). */
+int slab_laswp(int dtype, int batch, int ncols, int k1, int k2, int forward,
+               void* A, int64_t a_rs, int64_t a_cs, int64_t a_bs,
+               const int* ipiv, int64_t p_bs, void* stream);
+
+/* Batched LU with partial pivoting eliminating the first nelim columns of
+ * an nrows x ncols row-major matrix (lda, batch stride a_bs); pivot rows are
+ * restricted to rows < npiv.  nrows = ncols = nelim = npiv is a plain getrf
+ * (torch.linalg.lu_factor, src_solver/block_tridiag_solver_dense.py:157,169);
+ * nelim < ncols additionally forms L^{-1} P A12, and nrows > npiv carries
+ * bordered rows whose trailing block becomes the Schur complement. */
+int slab_getrf(int dtype, int batch, int nrows, int ncols, int nelim, int npiv,
+               void* A, int64_t lda, int64_t a_bs, int* ipiv, int64_t p_bs, int* info, void* stream);
+
+/* Batched solve with getrf factors: op(A) X = B (n x n, nrhs columns).
+ * Replaces torch.linalg.lu_solve (block_tridiag_solver_structured.py:58). */
+int slab_getrs(int dtype, int batch, int n, int nrhs, int trans,
+               const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv, int64_t p_bs,
+               void* B, int64_t ldb, int64_t b_bs, void* stream);
+
+/* Block-tridiagonal LU without inter-block pivoting over b independent
+ * chains of m blocks of size bs (row-major bs x bs blocks).  Overwrites
+ * diag_i with the packed LU of D_i - Lfac_i Ufac_i, sub_i with
+ * Lfac = sub_i U_i^{-1} and sup_i with Ufac = L_i^{-1} P_i^T sup_i.
+ * Block i of chain k is at ptr + k*bstride + i*istride.
+ * Replaces build_sweep (src_solver/block_tridiag_solver_dense.py:152-177)
+ * and the interface Schur recursion
+ * (src_solver/block_tridiag_solver_structured.py:34-76), which is the same
+ * factorization with b = 1. */
+int slab_btri_factor(int dtype, int b, int m, int bs,
+                     void* diag, int64_t d_bstride, int64_t d_istride,
+                     void* sub, int64_t l_bstride, int64_t l_istride,
+                     void* sup, int64_t u_bstride, int64_t u_istride,
+                     int* ipiv, int64_t p_bstride, int64_t p_istride, int* info, void* stream);
+
+/* Forward/backward block sweeps with the factors of slab_btri_factor, in
+ * place on f (b chains of m*bs rows, nrhs columns, row stride ldf, chain
+ * stride f_bstride); transpose solves A^T x = f.
+ * Replaces lusolve_sweep / lusolve_sweep_transpose
+ * (src_solver/block_tridiag_solver_dense.py:179-222) and
+ * HBS_BlockTridiagonal solve (block_tridiag_solver_structured.py:157-187). */
+int slab_btri_solve(int dtype, int b, int m, int bs,
+                    const void* diag, int64_t d_bstride, int64_t d_istride,
+                    const void* sub, int64_t l_bstride, int64_t l_istride,
+                    const void* sup, int64_t u_bstride, int64_t u_istride,
+                    const int* ipiv, int64_t p_bstride, int64_t p_istride,
+                    void* f, int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream);
+
+/* Leaf collocation rows for static condensation, all leaves in one launch:
+ * M = [A_cc | A_cx] (border = 0) or [A_cc | A_cx ; N_c | N_x] (border = 1)
+ * per leaf, from the scaled 1-D Chebyshev matrix D (p x p), Dsq = D@D and
+ * per-node coefficients (nboxes*p*p FP64 each; NULL = term absent; c11 and
+ * c22 required).  Nxtot (4(p-2) x ((p-2)^2 + 4(p-2)), FP64) is copied into
+ * the border rows.  Replaces get_Aloc_2d / Aloc_acc and the Acc/Acx
+ * gathers of src_disc/hps_parallel_leaf_ops.py:10-58. */
+int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D, const double* Dsq,
+                       const double* c11, const double* c22, const double* c12,
+                       const double* c1, const double* c2, const double* c,
+                       const double* Nxtot, void* M, int64_t ldm, int64_t m_bs, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLAB_HIP_H */

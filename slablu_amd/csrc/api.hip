@@ -1,0 +1,196 @@
+// extern "C" entry points of libslabhip (declared in include/slab_hip.h)
+// and the block-tridiagonal drivers built on the batched kernels.
+#include "slab_internal.h"
+#include "../../include/slab_hip.h"
+
+using namespace slab;
+
+namespace {
+
+inline int status(hipError_t e) { return e == hipSuccess ? 0 : -(1000 + (int)e); }
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <typename T>
+View<T> blk(void* base, int64_t bstride, int64_t istride, int i, int64_t ld) {
+  return View<T>{reinterpret_cast<T*>(base) + (int64_t)i * istride, ld, 1, bstride};
+}
+
+template <typename T>
+hipError_t btri_factor(int b, int m, int bs, void* diag, int64_t dbs, int64_t dis, void* sub, int64_t lbs,
+                       int64_t lis, void* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis,
+                       int* info, hipStream_t s) {
+  hipError_t e;
+  e = getrf<T>(b, bs, bs, bs, bs, blk<T>(diag, dbs, dis, 0, bs), ipiv, pbs, info, s);
+  if (e != hipSuccess) return e;
+  for (int i = 1; i < m; ++i) {
+    View<T> Dp = blk<T>(diag, dbs, dis, i - 1, bs);
+    View<T> Up = blk<T>(sup, ubs, uis, i - 1, bs);
+    View<T> Lp = blk<T>(sub, lbs, lis, i - 1, bs);
+    const int* pv = ipiv + (int64_t)(i - 1) * pis;
+    // Ufac = L_{i-1}^{-1} P^T sup_{i-1}
+    if ((e = laswp<T>(b, bs, 0, bs, true, Up, pv, pbs, s)) != hipSuccess) return e;
+    if ((e = trsm_left<T>(b, bs, bs, true, true, Dp, Up, s)) != hipSuccess) return e;
+    // Lfac = sub_{i-1} U_{i-1}^{-1}  <=>  U^T Lfac^T = sub^T
+    if ((e = trsm_left<T>(b, bs, bs, true, false, Dp.t(), Lp.t(), s)) != hipSuccess) return e;
+    View<T> Di = blk<T>(diag, dbs, dis, i, bs);
+    if ((e = gemm<T>(b, bs, bs, bs, T(-1), Lp, Up, T(1), Di, s)) != hipSuccess) return e;
+    if ((e = getrf<T>(b, bs, bs, bs, bs, Di, ipiv + (int64_t)i * pis, pbs, info, s)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t btri_solve(int b, int m, int bs, const void* diag, int64_t dbs, int64_t dis, const void* sub,
+                      int64_t lbs, int64_t lis, const void* sup, int64_t ubs, int64_t uis, const int* ipiv,
+                      int64_t pbs, int64_t pis, void* f, int64_t fbs, int64_t ldf, int nrhs, bool trans,
+                      hipStream_t s) {
+  auto D = [&](int i) { return blk<T>(const_cast<void*>(diag), dbs, dis, i, bs); };
+  auto L = [&](int i) { return blk<T>(const_cast<void*>(sub), lbs, lis, i, bs); };
+  auto U = [&](int i) { return blk<T>(const_cast<void*>(sup), ubs, uis, i, bs); };
+  auto F = [&](int i) { return View<T>{reinterpret_cast<T*>(f) + (int64_t)i * bs * ldf, ldf, 1, fbs}; };
+  auto P = [&](int i) { return ipiv + (int64_t)i * pis; };
+  hipError_t e;
+  if (!trans) {
+    for (int i = 0; i < m; ++i) {
+      if (i > 0 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), L(i - 1), F(i - 1), T(1), F(i), s)) != hipSuccess) return e;
+      if ((e = laswp<T>(b, nrhs, 0, bs, true, F(i), P(i), pbs, s)) != hipSuccess) return e;
+      if ((e = trsm_left<T>(b, bs, nrhs, true, true, D(i), F(i), s)) != hipSuccess) return e;
+    }
+    for (int i = m - 1; i >= 0; --i) {
+      if (i < m - 1 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), U(i), F(i + 1), T(1), F(i), s)) != hipSuccess) return e;
+      if ((e = trsm_left<T>(b, bs, nrhs, false, false, D(i), F(i), s)) != hipSuccess) return e;
+    }
+  } else {
+    // U_blk^T y = f : U_i^T y_i = f_i - Ufac_{i-1}^T y_{i-1}
+    for (int i = 0; i < m; ++i) {
+      if (i > 0 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), U(i - 1).t(), F(i - 1), T(1), F(i), s)) != hipSuccess)
+        return e;
+      if ((e = trsm_left<T>(b, bs, nrhs, true, false, D(i).t(), F(i), s)) != hipSuccess) return e;
+    }
+    // L_blk^T x = y : (P_i L_i)^T x_i = y_i - Lfac_i^T x_{i+1}
+    for (int i = m - 1; i >= 0; --i) {
+      if (i < m - 1 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), L(i).t(), F(i + 1), T(1), F(i), s)) != hipSuccess)
+        return e;
+      if ((e = trsm_left<T>(b, bs, nrhs, false, true, D(i).t(), F(i), s)) != hipSuccess) return e;
+      if ((e = laswp<T>(b, nrhs, 0, bs, false, F(i), P(i), pbs, s)) != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slab_version(void) { return 100; }
+
+int slab_gemm(int dtype, int batch, int m, int n, int k, double alpha, const void* A, int64_t a_rs, int64_t a_cs,
+              int64_t a_bs, const void* B, int64_t b_rs, int64_t b_cs, int64_t b_bs, double beta, void* C,
+              int64_t c_rs, int64_t c_cs, int64_t c_bs, void* stream) {
+  if (batch < 0 || m < 0 || n < 0 || k < 0) return -1;
+  if (dtype == SLAB_F64)
+    return status(gemm<double>(batch, m, n, k, alpha, view<double>(const_cast<void*>(A), a_rs, a_cs, a_bs),
+                               view<double>(const_cast<void*>(B), b_rs, b_cs, b_bs), beta,
+                               view<double>(C, c_rs, c_cs, c_bs), S(stream)));
+  if (dtype == SLAB_F32)
+    return status(gemm<float>(batch, m, n, k, (float)alpha, view<float>(const_cast<void*>(A), a_rs, a_cs, a_bs),
+                              view<float>(const_cast<void*>(B), b_rs, b_cs, b_bs), (float)beta,
+                              view<float>(C, c_rs, c_cs, c_bs), S(stream)));
+  return -1;
+}
+
+int slab_trsm(int dtype, int batch, int m, int n, int lower, int unit, const void* A, int64_t a_rs, int64_t a_cs,
+              int64_t a_bs, void* B, int64_t b_rs, int64_t b_cs, int64_t b_bs, void* stream) {
+  if (batch < 0 || m < 0 || n < 0) return -1;
+  if (dtype == SLAB_F64)
+    return status(trsm_left<double>(batch, m, n, lower != 0, unit != 0,
+                                     view<double>(const_cast<void*>(A), a_rs, a_cs, a_bs),
+                                     view<double>(B, b_rs, b_cs, b_bs), S(stream)));
+  if (dtype == SLAB_F32)
+    return status(trsm_left<float>(batch, m, n, lower != 0, unit != 0,
+                                    view<float>(const_cast<void*>(A), a_rs, a_cs, a_bs),
+                                    view<float>(B, b_rs, b_cs, b_bs), S(stream)));
+  return -1;
+}
+
+int slab_laswp(int dtype, int batch, int ncols, int k1, int k2, int forward, void* A, int64_t a_rs, int64_t a_cs,
+               int64_t a_bs, const int* ipiv, int64_t p_bs, void* stream) {
+  if (batch < 0 || ncols < 0 || k1 < 0 || k2 < k1) return -1;
+  if (dtype == SLAB_F64)
+    return status(laswp<double>(batch, ncols, k1, k2, forward != 0, view<double>(A, a_rs, a_cs, a_bs), ipiv, p_bs,
+                                S(stream)));
+  if (dtype == SLAB_F32)
+    return status(laswp<float>(batch, ncols, k1, k2, forward != 0, view<float>(A, a_rs, a_cs, a_bs), ipiv, p_bs,
+                               S(stream)));
+  return -1;
+}
+
+int slab_getrf(int dtype, int batch, int nrows, int ncols, int nelim, int npiv, void* A, int64_t lda, int64_t a_bs,
+               int* ipiv, int64_t p_bs, int* info, void* stream) {
+  if (batch < 0 || nelim < 0 || nelim > npiv || npiv > nrows || nelim > ncols || lda < ncols) return -1;
+  if (dtype == SLAB_F64)
+    return status(getrf<double>(batch, nrows, ncols, nelim, npiv, view<double>(A, lda, 1, a_bs), ipiv, p_bs, info,
+                                S(stream)));
+  if (dtype == SLAB_F32)
+    return status(getrf<float>(batch, nrows, ncols, nelim, npiv, view<float>(A, lda, 1, a_bs), ipiv, p_bs, info,
+                               S(stream)));
+  return -1;
+}
+
+int slab_getrs(int dtype, int batch, int n, int nrhs, int trans, const void* LU, int64_t lda, int64_t lu_bs,
+               const int* ipiv, int64_t p_bs, void* B, int64_t ldb, int64_t b_bs, void* stream) {
+  if (batch < 0 || n < 0 || nrhs < 0) return -1;
+  if (dtype == SLAB_F64)
+    return status(getrs<double>(batch, n, nrhs, trans != 0, view<double>(const_cast<void*>(LU), lda, 1, lu_bs), ipiv,
+                                p_bs, view<double>(B, ldb, 1, b_bs), S(stream)));
+  if (dtype == SLAB_F32)
+    return status(getrs<float>(batch, n, nrhs, trans != 0, view<float>(const_cast<void*>(LU), lda, 1, lu_bs), ipiv,
+                               p_bs, view<float>(B, ldb, 1, b_bs), S(stream)));
+  return -1;
+}
+
+int slab_btri_factor(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride, void* sub,
+                     int64_t l_bstride, int64_t l_istride, void* sup, int64_t u_bstride, int64_t u_istride,
+                     int* ipiv, int64_t p_bstride, int64_t p_istride, int* info, void* stream) {
+  if (b < 0 || m < 1 || bs < 1) return -1;
+  if (dtype == SLAB_F64)
+    return status(btri_factor<double>(b, m, bs, diag, d_bstride, d_istride, sub, l_bstride, l_istride, sup,
+                                      u_bstride, u_istride, ipiv, p_bstride, p_istride, info, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(btri_factor<float>(b, m, bs, diag, d_bstride, d_istride, sub, l_bstride, l_istride, sup,
+                                     u_bstride, u_istride, ipiv, p_bstride, p_istride, info, S(stream)));
+  return -1;
+}
+
+int slab_btri_solve(int dtype, int b, int m, int bs, const void* diag, int64_t d_bstride, int64_t d_istride,
+                    const void* sub, int64_t l_bstride, int64_t l_istride, const void* sup, int64_t u_bstride,
+                    int64_t u_istride, const int* ipiv, int64_t p_bstride, int64_t p_istride, void* f,
+                    int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream) {
+  if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs) return -1;
+  if (dtype == SLAB_F64)
+    return status(btri_solve<double>(b, m, bs, diag, d_bstride, d_istride, sub, l_bstride, l_istride, sup,
+                                     u_bstride, u_istride, ipiv, p_bstride, p_istride, f, f_bstride, ldf, nrhs,
+                                     transpose != 0, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(btri_solve<float>(b, m, bs, diag, d_bstride, d_istride, sub, l_bstride, l_istride, sup, u_bstride,
+                                    u_istride, ipiv, p_bstride, p_istride, f, f_bstride, ldf, nrhs, transpose != 0,
+                                    S(stream)));
+  return -1;
+}
+
+int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D, const double* Dsq,
+                       const double* c11, const double* c22, const double* c12, const double* c1, const double* c2,
+                       const double* c, const double* Nxtot, void* M, int64_t ldm, int64_t m_bs, void* stream) {
+  if (nboxes < 0 || p < 3 || !D || !Dsq || !c11 || !c22 || (border && !Nxtot)) return -1;
+  const int f = p - 2;
+  if (ldm < (int64_t)f * f + 4 * f) return -1;
+  if (dtype == SLAB_F64)
+    return status(leaf_assemble<double>(nboxes, p, border, D, Dsq, c11, c22, c12, c1, c2, c, Nxtot,
+                                        reinterpret_cast<double*>(M), ldm, m_bs, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(leaf_assemble<float>(nboxes, p, border, D, Dsq, c11, c22, c12, c1, c2, c, Nxtot,
+                                       reinterpret_cast<float*>(M), ldm, m_bs, S(stream)));
+  return -1;
+}
+
+}  // extern "C"

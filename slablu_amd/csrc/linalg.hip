@@ -1,0 +1,401 @@
+// Batched dense linear algebra for gfx950 (CDNA4): MFMA-tiled GEMM,
+// blocked triangular solves, row interchanges and a right-looking blocked
+// LU whose panel is factored in LDS with a workgroup-wide pivot search.
+// Every kernel takes strided views (row, column and batch strides) so that
+// transposes, sub-blocks and broadcast operands need no copies.
+#include "slab_internal.h"
+
+#include <climits>
+
+namespace slab {
+
+// ---------------------------------------------------------------------------
+// MFMA wrappers: 16x16x4 tiles.  Lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; accumulator element q of lane l is D[row(l, q)][l%16] with
+//   FP32 (v_mfma_f32_16x16x4f32): row = 4*(l/16) + q
+//   FP64 (v_mfma_f64_16x16x4f64): row = 4*q + l/16
+// (both measured on gfx950 with single-lane probes).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int q) { return 4 * q + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int q) { return 4 * (lane >> 4) + q; }
+};
+
+static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+static inline int grid_batch(int batch) { return batch < 65535 ? batch : 65535; }
+
+// ---------------------------------------------------------------------------
+// GEMM: 64x64 output tile per 256-thread workgroup, 4 waves in a 2x2
+// arrangement each owning 32x32 = 2x2 MFMA tiles; K staged through LDS in
+// chunks of 16.
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
+                                              T beta, View<T> C, int tiles_n) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[GBK][GBM + 1];
+  __shared__ T Bs[GBK][GBN + 1];
+  const int tile = blockIdx.x;
+  const int m0 = (tile / tiles_n) * GBM, n0 = (tile % tiles_n) * GBN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const bool a_rowmajor = (A.cs == 1);
+  const bool b_rowmajor = (B.cs == 1);
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int k0 = 0; k0 < k; k0 += GBK) {
+      for (int e = tid; e < GBM * GBK; e += 256) {
+        int r, kk;
+        if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
+        const int gr = m0 + r, gk = k0 + kk;
+        As[kk][r] = (gr < m && gk < k) ? *A.at(b, gr, gk) : T(0);
+      }
+      for (int e = tid; e < GBK * GBN; e += 256) {
+        int c, kk;
+        if (b_rowmajor) { kk = e / GBN; c = e % GBN; } else { kk = e % GBK; c = e / GBK; }
+        const int gc = n0 + c, gk = k0 + kk;
+        Bs[kk][c] = (gc < n && gk < k) ? *B.at(b, gk, gc) : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kb = 0; kb < GBK; kb += 4) {
+        T af[2], bf[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) af[i] = As[kb + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bf[j] = Bs[kb + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
+          if (row < m && col < n) {
+            T* cp = C.at(b, row, col);
+            T v = alpha * acc[i][j][q];
+            if (beta != T(0)) v += beta * (*cp);
+            *cp = v;
+          }
+        }
+      }
+  }
+}
+
+template <typename T>
+hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
+  if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) {
+    // C = beta * C: run the kernel with a zero-length K loop.
+    k = 0;
+  }
+  const int tn = ceil_div(n, GBN), tm = ceil_div(m, GBM);
+  dim3 grid(tm * tn, grid_batch(batch));
+  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solve of a <= 32 x 32 diagonal block: one thread per right-hand
+// side column, the block in LDS, the solution in registers.
+// ---------------------------------------------------------------------------
+constexpr int TB = 32;
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_trsm_diag(int batch, int n, int d0, int tb, int lower, int unit, View<T> A,
+                                                   View<T> B) {
+  __shared__ T Ls[TB][TB + 1];
+  const int tid = threadIdx.x;
+  const int col = blockIdx.x * 128 + tid;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    for (int e = tid; e < TB * TB; e += 128) {
+      const int r = e / TB, c = e % TB;
+      Ls[r][c] = (r < tb && c < tb) ? *A.at(b, d0 + r, d0 + c) : T(0);
+    }
+    __syncthreads();
+    if (col < n) {
+      T x[TB];
+#pragma unroll
+      for (int r = 0; r < TB; ++r) x[r] = (r < tb) ? *B.at(b, d0 + r, col) : T(0);
+      if (lower) {
+#pragma unroll
+        for (int i = 0; i < TB; ++i) {
+          if (i < tb) {
+            T v = x[i];
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+              if (j < i) v -= Ls[i][j] * x[j];
+            if (!unit) v = v / Ls[i][i];
+            x[i] = v;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = TB - 1; i >= 0; --i) {
+          if (i < tb) {
+            T v = x[i];
+#pragma unroll
+            for (int j = 0; j < TB; ++j)
+              if (j > i && j < tb) v -= Ls[i][j] * x[j];
+            if (!unit) v = v / Ls[i][i];
+            x[i] = v;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < TB; ++r)
+        if (r < tb) *B.at(b, d0 + r, col) = x[r];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+hipError_t trsm_left(int batch, int m, int n, bool lower, bool unit, View<T> A, View<T> B, hipStream_t s) {
+  if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  dim3 grid(ceil_div(n, 128), grid_batch(batch));
+  hipError_t e = hipSuccess;
+  if (lower) {
+    for (int d0 = 0; d0 < m; d0 += TB) {
+      const int tb = (m - d0) < TB ? (m - d0) : TB;
+      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(128), 0, s, batch, n, d0, tb, 1, unit ? 1 : 0, A, B);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if (d0 + tb < m) {
+        e = gemm<T>(batch, m - d0 - tb, n, tb, T(-1), A.sub(d0 + tb, d0), B.sub(d0, 0), T(1), B.sub(d0 + tb, 0), s);
+        if (e != hipSuccess) return e;
+      }
+    }
+  } else {
+    const int last = ((m - 1) / TB) * TB;
+    for (int d0 = last; d0 >= 0; d0 -= TB) {
+      const int tb = (m - d0) < TB ? (m - d0) : TB;
+      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(128), 0, s, batch, n, d0, tb, 0, unit ? 1 : 0, A, B);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if (d0 > 0) {
+        e = gemm<T>(batch, d0, n, tb, T(-1), A.sub(0, d0), B.sub(d0, 0), T(1), B, s);
+        if (e != hipSuccess) return e;
+      }
+    }
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Row interchanges: one thread per column applies the pivot sequence.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(int batch, int ncols, int k1, int k2, int forward, View<T> A,
+                                               const int* ipiv, int64_t pbs, int skip0, int skip1) {
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= ncols || (col >= skip0 && col < skip1)) return;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    const int* pv = ipiv + (int64_t)b * pbs;
+    if (forward) {
+      for (int k = k1; k < k2; ++k) {
+        const int p = pv[k] - 1;
+        if (p != k) {
+          T* x = A.at(b, k, col);
+          T* y = A.at(b, p, col);
+          T t = *x; *x = *y; *y = t;
+        }
+      }
+    } else {
+      for (int k = k2 - 1; k >= k1; --k) {
+        const int p = pv[k] - 1;
+        if (p != k) {
+          T* x = A.at(b, k, col);
+          T* y = A.at(b, p, col);
+          T t = *x; *x = *y; *y = t;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+hipError_t laswp(int batch, int ncols, int k1, int k2, bool forward, View<T> A, const int* ipiv, int64_t pbs,
+                 hipStream_t s) {
+  if (batch <= 0 || ncols <= 0 || k2 <= k1) return hipSuccess;
+  dim3 grid(ceil_div(ncols, 256), grid_batch(batch));
+  hipLaunchKernelGGL(k_laswp<T>, grid, dim3(256), 0, s, batch, ncols, k1, k2, forward ? 1 : 0, A, ipiv, pbs, 0, 0);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// LU panel: rows k0..nrows-1, columns k0..k0+w-1 of one matrix per
+// workgroup, held column-major in LDS.  Pivot search: per-thread scan,
+// wavefront butterfly reduction, then across the 4 waves; ties go to the
+// smallest row (LAPACK idamax).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T tabs(T v) { return v < T(0) ? -v : v; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_getf2_panel(int batch, int nrows, int k0, int w, int npiv, View<T> A,
+                                                     int* ipiv, int64_t pbs, int* info) {
+  extern __shared__ unsigned char smem_raw[];
+  T* Ps = reinterpret_cast<T*>(smem_raw);
+  __shared__ T red_v[4];
+  __shared__ int red_i[4];
+  __shared__ int s_piv;
+  const int R = nrows - k0;
+  const int Rp = npiv - k0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    for (int e = tid; e < R * w; e += 256) {
+      const int r = e / w, c = e % w;
+      Ps[c * R + r] = *A.at(b, k0 + r, k0 + c);
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+      T best = T(-1);
+      int bi = INT_MAX;
+      for (int r = j + tid; r < Rp; r += 256) {
+        const T v = tabs(Ps[j * R + r]);
+        if (v > best) { best = v; bi = r; }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+      __syncthreads();
+      if (tid == 0) {
+        T bv = red_v[0];
+        int bx = red_i[0];
+        for (int q = 1; q < 4; ++q)
+          if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) { bv = red_v[q]; bx = red_i[q]; }
+        if (bx < j || bx >= Rp) bx = j;  // no valid candidate (e.g. NaN column): keep the diagonal
+        s_piv = bx;
+        ipiv[(int64_t)b * pbs + k0 + j] = k0 + bx + 1;
+      }
+      __syncthreads();
+      const int pv = s_piv;
+      if (pv != j && tid < w) {
+        T t = Ps[tid * R + j];
+        Ps[tid * R + j] = Ps[tid * R + pv];
+        Ps[tid * R + pv] = t;
+      }
+      __syncthreads();
+      const T piv = Ps[j * R + j];
+      if (piv != T(0)) {
+        for (int r = j + 1 + tid; r < R; r += 256) Ps[j * R + r] = Ps[j * R + r] / piv;
+      } else if (tid == 0 && info != nullptr && info[b] == 0) {
+        info[b] = k0 + j + 1;
+      }
+      __syncthreads();
+      const int nr = R - j - 1, nc = w - j - 1;
+      if (nr > 0 && nc > 0) {
+        for (int e = tid; e < nr * nc; e += 256) {
+          const int c = j + 1 + e / nr, r = j + 1 + e % nr;
+          Ps[c * R + r] -= Ps[j * R + r] * Ps[c * R + j];
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < R * w; e += 256) {
+      const int r = e / w, c = e % w;
+      *A.at(b, k0 + r, k0 + c) = Ps[c * R + r];
+    }
+    __syncthreads();
+  }
+}
+
+static const size_t kPanelLdsBudget = 128 * 1024;
+
+template <typename T>
+hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs, int* info,
+                 hipStream_t s) {
+  if (batch <= 0 || nelim <= 0) return hipSuccess;
+  if (nelim > npiv || npiv > nrows || nelim > ncols) return hipErrorInvalidValue;
+  int nb = 32;
+  while (nb > 4 && (size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) nb >>= 1;
+  if ((size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) return hipErrorInvalidValue;
+  static bool attr_set[2] = {false, false};
+  const int ti = sizeof(T) == 8 ? 0 : 1;
+  hipError_t e;
+  if (!attr_set[ti]) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getf2_panel<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPanelLdsBudget);
+    if (e != hipSuccess) return e;
+    attr_set[ti] = true;
+  }
+  for (int k0 = 0; k0 < nelim; k0 += nb) {
+    const int w = (nelim - k0) < nb ? (nelim - k0) : nb;
+    const size_t lds = (size_t)(nrows - k0) * w * sizeof(T);
+    hipLaunchKernelGGL(k_getf2_panel<T>, dim3(grid_batch(batch)), dim3(256), lds, s, batch, nrows, k0, w, npiv, A,
+                       ipiv, pbs, info);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // interchanges on every column outside the panel (the panel kernel
+    // already swapped its own columns), then U12 = L11^{-1} A12
+    dim3 g2(ceil_div(ncols, 256), grid_batch(batch));
+    hipLaunchKernelGGL(k_laswp<T>, g2, dim3(256), 0, s, batch, ncols, k0, k0 + w, 1, A, ipiv, pbs, k0, k0 + w);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int mr = nrows - k0 - w, nr = ncols - k0 - w;
+    if (nr > 0) {
+      dim3 g3(ceil_div(nr, 128), grid_batch(batch));
+      hipLaunchKernelGGL(k_trsm_diag<T>, g3, dim3(128), 0, s, batch, nr, 0, w, 1, 1, A.sub(k0, k0), A.sub(k0, k0 + w));
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (mr > 0 && nr > 0) {
+      e = gemm<T>(batch, mr, nr, w, T(-1), A.sub(k0 + w, k0), A.sub(k0, k0 + w), T(1), A.sub(k0 + w, k0 + w), s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
+                 hipStream_t s) {
+  hipError_t e;
+  if (!trans) {
+    if ((e = laswp<T>(batch, nrhs, 0, n, true, B, ipiv, pbs, s)) != hipSuccess) return e;
+    if ((e = trsm_left<T>(batch, n, nrhs, true, true, LU, B, s)) != hipSuccess) return e;
+    return trsm_left<T>(batch, n, nrhs, false, false, LU, B, s);
+  }
+  if ((e = trsm_left<T>(batch, n, nrhs, true, false, LU.t(), B, s)) != hipSuccess) return e;
+  if ((e = trsm_left<T>(batch, n, nrhs, false, true, LU.t(), B, s)) != hipSuccess) return e;
+  return laswp<T>(batch, nrhs, 0, n, false, B, ipiv, pbs, s);
+}
+
+#define SLAB_INSTANTIATE(T)                                                                                        \
+  template hipError_t gemm<T>(int, int, int, int, T, View<T>, View<T>, T, View<T>, hipStream_t);                  \
+  template hipError_t trsm_left<T>(int, int, int, bool, bool, View<T>, View<T>, hipStream_t);                     \
+  template hipError_t laswp<T>(int, int, int, int, bool, View<T>, const int*, int64_t, hipStream_t);              \
+  template hipError_t getrf<T>(int, int, int, int, int, View<T>, int*, int64_t, int*, hipStream_t);               \
+  template hipError_t getrs<T>(int, int, int, bool, View<T>, const int*, int64_t, View<T>, hipStream_t);
+SLAB_INSTANTIATE(double)
+SLAB_INSTANTIATE(float)
+
+}  // namespace slab

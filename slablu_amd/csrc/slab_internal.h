@@ -1,0 +1,62 @@
+// Internal declarations shared by the HIP translation units of libslabhip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace slab {
+
+// A strided matrix view: element (i, j) of batch b lives at
+//   base + b * bs + i * rs + j * cs.
+// Transposition is a swap of (rs, cs); every kernel takes views, so op(A)
+// never needs a copy.
+template <typename T>
+struct View {
+  T* p;
+  int64_t rs, cs, bs;
+  __host__ __device__ T* at(int64_t b, int64_t i, int64_t j) const { return p + b * bs + i * rs + j * cs; }
+  __host__ __device__ View<T> sub(int64_t i, int64_t j) const { return View<T>{p + i * rs + j * cs, rs, cs, bs}; }
+  __host__ __device__ View<T> t() const { return View<T>{p, cs, rs, bs}; }
+};
+
+template <typename T>
+inline View<T> view(void* p, int64_t rs, int64_t cs, int64_t bs) {
+  return View<T>{reinterpret_cast<T*>(p), rs, cs, bs};
+}
+
+// C(b) = alpha * A(b) @ B(b) + beta * C(b);  A m x k, B k x n, C m x n.
+template <typename T>
+hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C,
+                hipStream_t s);
+
+// B(b) <- op(A(b))^{-1} B(b) for a triangular m x m A; lower/upper refer to
+// the view as given (pass A.t() for a transposed solve).
+template <typename T>
+hipError_t trsm_left(int batch, int m, int n, bool lower, bool unit, View<T> A, View<T> B, hipStream_t s);
+
+// Row interchanges rows k1..k2-1 of an (nrows x ncols) view with the 1-based
+// pivots ipiv[k] (forward: k ascending; backward: descending).
+template <typename T>
+hipError_t laswp(int batch, int ncols, int k1, int k2, bool forward, View<T> A, const int* ipiv, int64_t pbs,
+                 hipStream_t s);
+
+// Right-looking blocked LU with partial pivoting eliminating the first
+// `nelim` columns of an nrows x ncols row-major matrix; pivot candidates are
+// restricted to rows < npiv.  ipiv is 1-based (LAPACK).  info[b] receives the
+// 1-based index of the first exactly-zero pivot (0 if none) when non-null.
+template <typename T>
+hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs,
+                 int* info, hipStream_t s);
+
+// Solve op(A) X = B with the factors of getrf (square n x n).
+template <typename T>
+hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
+                 hipStream_t s);
+
+}  // namespace slab
+
+namespace slab {
+template <typename T>
+hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const double* Dsq, const double* c11,
+                         const double* c22, const double* c12, const double* c1, const double* c2, const double* c,
+                         const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s);
+}  // namespace slab

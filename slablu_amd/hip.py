@@ -1,0 +1,194 @@
+"""ctypes binding of libslabhip.so (include/slab_hip.h) for torch tensors.
+
+Every function takes torch tensors that live on the GPU and passes their
+data pointers and strides to the C ABI; all launches are asynchronous on the
+current torch stream.  The library is mandatory: if it is missing, importing
+this module raises, and there is no CPU fallback anywhere in the product
+path.
+"""
+import ctypes
+import os
+
+import torch
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libslabhip.so")
+
+F64, F32 = 0, 1
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_dbl = ctypes.c_double
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libslabhip.so is not built (%s); build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python slablu_amd/_build.py`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "slab_version": ([], _int),
+        "slab_gemm": ([_int, _int, _int, _int, _int, _dbl, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _dbl, _vp,
+                       _i64, _i64, _i64, _vp], _int),
+        "slab_trsm": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _vp], _int),
+        "slab_laswp": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _vp], _int),
+        "slab_getrf": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _vp], _int),
+        "slab_getrs": ([_int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
+        "slab_btri_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
+                              _i64, _vp, _vp], _int),
+        "slab_btri_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
+                             _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
+        "slab_leaf_assemble": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
+                                _vp], _int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+LIB = _load()
+
+
+def _check(rc, name):
+    if rc != 0:
+        raise RuntimeError("%s failed with status %d" % (name, rc))
+
+
+def _dt(t):
+    if t.dtype == torch.float64:
+        return F64
+    if t.dtype == torch.float32:
+        return F32
+    raise TypeError("unsupported dtype %s" % t.dtype)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _mat(t):
+    """(batch, rows, cols) strides of a 2-D or 3-D tensor view."""
+    if t.dim() == 2:
+        return 1, t.stride(0), t.stride(1), 0
+    if t.dim() == 3:
+        return t.shape[0], t.stride(1), t.stride(2), t.stride(0)
+    raise ValueError("expected a 2-D or 3-D tensor, got shape %s" % (tuple(t.shape),))
+
+
+def _require(cond, msg):
+    if not cond:
+        raise ValueError(msg)
+
+
+def version():
+    return LIB.slab_version()
+
+
+def gemm(A, B, C, alpha=1.0, beta=0.0):
+    """C = alpha A@B + beta C for (batched) strided views; 2-D operands are
+    broadcast over the batch of C."""
+    bc, crs, ccs, cbs = _mat(C)
+    ba, ars, acs, abs_ = _mat(A)
+    bb, brs, bcs, bbs = _mat(B)
+    m, n = C.shape[-2], C.shape[-1]
+    k = A.shape[-1]
+    _require(A.shape[-2] == m and B.shape[-2] == k and B.shape[-1] == n, "gemm shape mismatch")
+    _require(ba in (1, bc) and bb in (1, bc), "gemm batch mismatch")
+    _require(A.dtype == B.dtype == C.dtype and A.is_cuda and B.is_cuda and C.is_cuda, "gemm dtype/device mismatch")
+    if ba == 1 and A.dim() == 3:
+        abs_ = 0
+    if bb == 1 and B.dim() == 3:
+        bbs = 0
+    _check(LIB.slab_gemm(_dt(C), bc, m, n, k, float(alpha), _p(A), ars, acs, abs_, _p(B), brs, bcs, bbs,
+                         float(beta), _p(C), crs, ccs, cbs, _stream()), "slab_gemm")
+    return C
+
+
+def trsm(A, B, lower, unit):
+    """B <- A^{-1} B in place (A triangular as viewed)."""
+    bb, brs, bcs, bbs = _mat(B)
+    ba, ars, acs, abs_ = _mat(A)
+    m, n = B.shape[-2], B.shape[-1]
+    _require(A.shape[-1] == m and A.shape[-2] == m, "trsm shape mismatch")
+    _require(ba in (1, bb), "trsm batch mismatch")
+    if ba == 1 and A.dim() == 3:
+        abs_ = 0
+    _check(LIB.slab_trsm(_dt(B), bb, m, n, int(lower), int(unit), _p(A), ars, acs, abs_, _p(B), brs, bcs, bbs,
+                         _stream()), "slab_trsm")
+    return B
+
+
+def laswp(A, ipiv, k1, k2, forward=True):
+    bb, rs, cs, bs = _mat(A)
+    pbs = ipiv.stride(0) if ipiv.dim() == 2 else 0
+    _check(LIB.slab_laswp(_dt(A), bb, A.shape[-1], k1, k2, int(forward), _p(A), rs, cs, bs, _p(ipiv), pbs,
+                          _stream()), "slab_laswp")
+    return A
+
+
+def getrf(A, ipiv, nelim=None, npiv=None, info=None):
+    """In-place batched LU (row-major, unit column stride) of A (batch, nrows,
+    ncols) eliminating the first nelim columns; ipiv (batch, >= nelim) int32."""
+    _require(A.dim() == 3 and A.stride(2) == 1, "getrf expects a (batch, rows, cols) row-major tensor")
+    b, nr, nc = A.shape
+    nelim = min(nr, nc) if nelim is None else nelim
+    npiv = nr if npiv is None else npiv
+    _require(ipiv.dtype == torch.int32 and ipiv.shape[0] == b and ipiv.shape[1] >= nelim, "ipiv shape/dtype")
+    _check(LIB.slab_getrf(_dt(A), b, nr, nc, nelim, npiv, _p(A), A.stride(1), A.stride(0), _p(ipiv), ipiv.stride(0),
+                          _p(info), _stream()), "slab_getrf")
+    return A
+
+
+def getrs(LU, ipiv, B, trans=False):
+    _require(LU.dim() == 3 and B.dim() == 3 and LU.stride(2) == 1 and B.stride(2) == 1, "getrs layouts")
+    b, n, _ = LU.shape
+    _check(LIB.slab_getrs(_dt(B), b, n, B.shape[2], int(trans), _p(LU), LU.stride(1), LU.stride(0), _p(ipiv),
+                          ipiv.stride(0), _p(B), B.stride(1), B.stride(0), _stream()), "slab_getrs")
+    return B
+
+
+def btri_factor(diag, sub, sup, ipiv, info=None):
+    """Block-tridiagonal LU in place: diag (b, m, bs, bs), sub/sup (b, m-1, bs,
+    bs), ipiv (b, m, bs) int32; blocks must be contiguous row-major."""
+    b, m, bs, _ = diag.shape
+    for t in (diag, sub, sup):
+        _require(t.stride(-1) == 1 and t.stride(-2) == bs, "btri blocks must be contiguous row-major")
+    _require(ipiv.stride(-1) == 1, "ipiv layout")
+    _check(LIB.slab_btri_factor(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub),
+                                sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
+                                sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(info),
+                                _stream()), "slab_btri_factor")
+
+
+def btri_solve(diag, sub, sup, ipiv, f, transpose=False):
+    """In-place sweeps on f (b, m*bs, nrhs) (row-major rows)."""
+    b, m, bs, _ = diag.shape
+    _require(f.dim() == 3 and f.shape[0] == b and f.shape[1] == m * bs and f.stride(2) == 1, "btri rhs layout")
+    _check(LIB.slab_btri_solve(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub),
+                               sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
+                               sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(f),
+                               f.stride(0), f.stride(1), f.shape[2], int(transpose), _stream()), "slab_btri_solve")
+    return f
+
+
+def leaf_assemble(M, p, border, D, Dsq, coefs, Nxtot):
+    """Fill M (nboxes, nrows, >= n_i + n_x) with the condensation rows of every
+    leaf; coefs: dict of FP64 (nboxes*p*p) tensors for c11, c22 and optionally
+    c12, c1, c2, c."""
+    _require(M.dim() == 3 and M.stride(2) == 1, "leaf M layout")
+    for k in ("c11", "c22"):
+        _require(coefs.get(k) is not None, "coefficient %s required" % k)
+    args = [coefs.get(k) for k in ("c11", "c22", "c12", "c1", "c2", "c")]
+    for a in args:
+        _require(a is None or (a.dtype == torch.float64 and a.is_contiguous() and a.numel() == M.shape[0] * p * p),
+                 "coefficient arrays must be contiguous FP64 of nboxes*p*p")
+    _check(LIB.slab_leaf_assemble(_dt(M), M.shape[0], p, int(border), _p(D), _p(Dsq), *[_p(a) for a in args],
+                                  _p(Nxtot), _p(M), M.stride(1), M.stride(0), _stream()), "slab_leaf_assemble")
+    return M

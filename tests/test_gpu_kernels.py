@@ -1,0 +1,234 @@
+"""Parity of the batched HIP kernels (C ABI through slablu_amd.hip) against
+FP64 CPU references (torch / scipy on the host)."""
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-12, torch.float32: 2e-5}
+
+
+def _hip():
+    from slablu_amd import hip
+    return hip
+
+
+def _rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return (torch.linalg.norm(a - b) / max(torch.linalg.norm(b), 1e-300)).item()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 45), (200, 3, 97)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm(dtype, m, n, k, ta, tb):
+    hip = _hip()
+    g = torch.Generator().manual_seed(m * 1000 + n * 10 + k)
+    batch = 3
+    A0 = torch.randn(batch, k, m, generator=g, dtype=torch.float64) if ta else torch.randn(batch, m, k, generator=g, dtype=torch.float64)
+    B0 = torch.randn(batch, n, k, generator=g, dtype=torch.float64) if tb else torch.randn(batch, k, n, generator=g, dtype=torch.float64)
+    C0 = torch.randn(batch, m, n, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype)
+    B = B0.to("cuda", dtype)
+    C = C0.to("cuda", dtype)
+    Av = A.transpose(1, 2) if ta else A
+    Bv = B.transpose(1, 2) if tb else B
+    hip.gemm(Av, Bv, C, alpha=-0.5, beta=2.0)
+    ref = -0.5 * (A0.transpose(1, 2) if ta else A0) @ (B0.transpose(1, 2) if tb else B0) + 2.0 * C0
+    torch.cuda.synchronize()
+    assert _rel(C, ref) < TOL[dtype]
+
+
+def test_gemm_broadcast_and_zero_beta():
+    hip = _hip()
+    g = torch.Generator().manual_seed(1)
+    A0 = torch.randn(40, 50, generator=g, dtype=torch.float64)
+    B0 = torch.randn(5, 50, 30, generator=g, dtype=torch.float64)
+    C = torch.full((5, 40, 30), float("nan"), dtype=torch.float64, device="cuda")
+    hip.gemm(A0.cuda(), B0.cuda(), C, alpha=1.0, beta=0.0)
+    torch.cuda.synchronize()
+    assert _rel(C, A0 @ B0) < 1e-13
+
+
+def _tri(b, m, lower, g, dtype):
+    A = torch.randn(b, m, m, generator=g, dtype=torch.float64) * 0.1
+    A = torch.tril(A) if lower else torch.triu(A)
+    A = A + torch.diag_embed(1.0 + torch.rand(b, m, generator=g, dtype=torch.float64))
+    return A
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,n", [(1, 1), (20, 7), (45, 130), (100, 33)])
+@pytest.mark.parametrize("lower", [True, False])
+@pytest.mark.parametrize("unit", [True, False])
+@pytest.mark.parametrize("transA", [False, True])
+def test_trsm(dtype, m, n, lower, unit, transA):
+    hip = _hip()
+    g = torch.Generator().manual_seed(m + 7 * n)
+    b = 2
+    # view lower/upper as given after the optional transpose
+    A0 = _tri(b, m, lower != transA, g, dtype)
+    Aview0 = A0.transpose(1, 2) if transA else A0
+    B0 = torch.randn(b, m, n, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype)
+    Av = A.transpose(1, 2) if transA else A
+    X = B0.to("cuda", dtype).clone()
+    hip.trsm(Av, X, lower=lower, unit=unit)
+    Aref = Aview0.clone()
+    if unit:
+        Aref = Aref - torch.diag_embed(torch.diagonal(Aref, dim1=1, dim2=2)) + torch.eye(m, dtype=torch.float64)
+    ref = torch.linalg.solve_triangular(Aref, B0, upper=not lower)
+    torch.cuda.synchronize()
+    assert _rel(X, ref) < 50 * TOL[dtype]
+
+
+def _lu_check(Af, LUf, piv, n):
+    """||P L U - A|| / ||A|| for one square matrix (numpy)."""
+    L = np.tril(LUf, -1) + np.eye(n)
+    U = np.triu(LUf)
+    PA = Af.copy()
+    for i in range(n):
+        p = piv[i] - 1
+        if p != i:
+            PA[[i, p]] = PA[[p, i]]
+    return np.linalg.norm(L @ U - PA) / np.linalg.norm(Af)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [1, 5, 20, 33, 100, 257])
+def test_getrf_square_matches_lapack(dtype, n):
+    hip = _hip()
+    g = torch.Generator().manual_seed(n)
+    b = 4
+    A0 = torch.randn(b, n, n, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype).contiguous()
+    ipiv = torch.zeros(b, n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(b, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv, info=info)
+    torch.cuda.synchronize()
+    LU = A.double().cpu().numpy()
+    pv = ipiv.cpu().numpy()
+    assert (info.cpu() == 0).all()
+    for k in range(b):
+        lu_ref, piv_ref = scipy.linalg.lu_factor(A0[k].numpy())
+        if dtype == torch.float64:
+            assert np.array_equal(pv[k] - 1, piv_ref)
+            assert np.abs(LU[k] - lu_ref).max() < 1e-10 * max(1, np.abs(lu_ref).max())
+        assert _lu_check(A0[k].to(dtype).double().numpy(), LU[k], pv[k], n) < 10 * TOL[dtype] * max(1, n / 10)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_getrf_rectangular_and_bordered(dtype):
+    """[A11 A12; A21 A22] with pivots restricted to the first ni rows:
+    A12 <- L^{-1} P A12 and A22 <- A22 - A21 A11^{-1} A12."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(3)
+    b, ni, nx = 3, 100, 40
+    M0 = torch.randn(b, ni + nx, ni + nx, generator=g, dtype=torch.float64)
+    M0[:, :ni, :ni] += 5 * torch.eye(ni, dtype=torch.float64)
+    for bordered in (False, True):
+        nrows = ni + nx if bordered else ni
+        M = M0[:, :nrows].to("cuda", dtype).contiguous()
+        ipiv = torch.zeros(b, ni, dtype=torch.int32, device="cuda")
+        hip.getrf(M, ipiv, nelim=ni, npiv=ni)
+        torch.cuda.synchronize()
+        Mc = M.double().cpu()
+        A11, A12 = M0[:, :ni, :ni], M0[:, :ni, ni:]
+        # U^{-1} (L^{-1} P A12) = A11^{-1} A12
+        X = torch.linalg.solve_triangular(torch.triu(Mc[:, :ni, :ni]), Mc[:, :ni, ni:], upper=True)
+        assert _rel(X, torch.linalg.solve(A11, A12)) < 100 * TOL[dtype]
+        if bordered:
+            S = M0[:, ni:, ni:] - M0[:, ni:, :ni] @ torch.linalg.solve(A11, A12)
+            assert _rel(Mc[:, ni:, ni:], S) < 100 * TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("trans", [False, True])
+def test_getrs(dtype, trans):
+    hip = _hip()
+    g = torch.Generator().manual_seed(11)
+    b, n, nrhs = 3, 70, 9
+    A0 = torch.randn(b, n, n, generator=g, dtype=torch.float64) + 4 * torch.eye(n, dtype=torch.float64)
+    B0 = torch.randn(b, n, nrhs, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype).contiguous()
+    ipiv = torch.zeros(b, n, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv)
+    X = B0.to("cuda", dtype).contiguous()
+    hip.getrs(A, ipiv, X, trans=trans)
+    ref = torch.linalg.solve(A0.transpose(1, 2) if trans else A0, B0)
+    torch.cuda.synchronize()
+    assert _rel(X, ref) < 100 * TOL[dtype]
+
+
+def _btri_dense(diag, sub, sup):
+    b, m, bs, _ = diag.shape
+    M = torch.zeros(b, m * bs, m * bs, dtype=torch.float64)
+    for i in range(m):
+        M[:, i * bs:(i + 1) * bs, i * bs:(i + 1) * bs] = diag[:, i]
+        if i < m - 1:
+            M[:, (i + 1) * bs:(i + 2) * bs, i * bs:(i + 1) * bs] = sub[:, i]
+            M[:, i * bs:(i + 1) * bs, (i + 1) * bs:(i + 2) * bs] = sup[:, i]
+    return M
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (1, 4, 100), (4, 2, 37)])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_btri_factor_solve(dtype, b, m, bs, transpose):
+    hip = _hip()
+    g = torch.Generator().manual_seed(b * 100 + m * 10 + bs)
+    diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
+    sub0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    sup0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    nrhs = 6
+    f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
+    diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
+    ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
+    hip.btri_factor(diag, sub, sup, ipiv)
+    f = f0.to("cuda", dtype).contiguous()
+    hip.btri_solve(diag, sub, sup, ipiv, f, transpose=transpose)
+    M = _btri_dense(diag0, sub0[:, :m - 1], sup0[:, :m - 1])
+    ref = torch.linalg.solve(M.transpose(1, 2) if transpose else M, f0)
+    torch.cuda.synchronize()
+    assert _rel(f, ref) < 200 * TOL[dtype]
+
+
+@pytest.mark.parametrize("p", [10, 16, 22])
+@pytest.mark.parametrize("general", [False, True])
+def test_leaf_assemble_bit_exact(p, general):
+    from oracle import hps_oracle as O
+    hip = _hip()
+    rng = np.random.default_rng(p)
+    a = 0.013
+    ones = lambda xx: np.ones(xx.shape[0])
+    if general:
+        pdo = O.PDO(lambda xx: 1 + 0.3 * np.sin(7 * xx[:, 0]), lambda xx: 2 + xx[:, 1] ** 2,
+                    c12=lambda xx: 0.1 * xx[:, 0] * xx[:, 1], c1=lambda xx: np.cos(xx[:, 1]),
+                    c2=lambda xx: -xx[:, 0], c=lambda xx: -400 * (1 + 0.2 * np.cos(9 * xx[:, 0])))
+    else:
+        pdo = O.PDO(ones, ones, c=lambda xx: -300.0 * np.ones(xx.shape[0]))
+    hps = O.HPSOracle(pdo, np.array([[0.0, 6 * a], [0.0, 4 * a]]), a, p)
+    nb = hps.nboxes
+    A = hps.aloc(np.arange(nb))
+    Jc, Jx = hps.JJ["Jc"], hps.JJ["Jx"]
+    Jtot = np.hstack((Jc, Jx))
+    xx = hps.grid_xx.reshape(-1, 2)
+    coefs = {}
+    for name in ("c11", "c22", "c12", "c1", "c2", "c"):
+        fn = getattr(pdo, name)
+        if fn is not None:
+            coefs[name] = torch.tensor(np.asarray(fn(xx), dtype=np.float64).reshape(-1), device="cuda")
+    D = torch.tensor(hps.D, device="cuda")
+    Dsq = torch.tensor(hps.Dsq, device="cuda")
+    Nxtot = torch.tensor(np.ascontiguousarray(hps.Nx[:, Jtot]), device="cuda")
+    ni, nx = len(Jc), len(Jx)
+    M = torch.zeros(nb, ni + nx, ni + nx, dtype=torch.float64, device="cuda")
+    hip.leaf_assemble(M, p, 1, D, Dsq, coefs, Nxtot)
+    torch.cuda.synchronize()
+    Mc = M.cpu().numpy()
+    ref = A[:, Jc][:, :, Jtot]
+    assert np.array_equal(Mc[:, :ni], ref)
+    assert np.array_equal(Mc[:, ni:], np.broadcast_to(hps.Nx[:, Jtot], (nb, nx, ni + nx)))
