@@ -1,0 +1,197 @@
+"""SlabLU factor+solve throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1000] [--p 22]
+
+Workload (default): Helmholtz -Lap u - kh^2 (1 - b(x)) u = 0 on the unit
+square, n=1000, p=22 (50 x 50 leaves, N = 1,102,000 DOFs), 10 points per
+wavelength (kh = 630.31), b(x) = Gaussian bumps (amplitude 1, width 0.005)
+on a 0.04 lattice with one row and one column removed (L-shaped waveguide),
+Dirichlet data from the free-space Green's function.  A step is one full
+factor + solve of the mixed-precision path (FP32 leaf/slab stages, FP64
+interface LU, FP64-residual refinement of the full discrete system to a
+correction below 1e-11).  The FP64 path is timed on the same problem and
+its solution is the reference for the error of the mixed solution.
+
+Prints one JSON line (metric, value = DOF/s, ...), with the roofline of the
+dominant stage (leaf condensation) measured with HIP events on the stream
+the kernels run on, and a CPU baseline: the oracle on a bounded sample of
+the same workload on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "factor+solve DOF/s at N≈1M p=22: mixed+refined vs FP64 at 1e-8 rel err"
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+
+
+def leaf_flops(p):
+    """Algorithmic FLOPs of one leaf condensation: LU of the (p-2)^2 interior
+    block, its solve against the 4(p-2) face columns and the DtN product."""
+    ni, nb = (p - 2) ** 2, 4 * (p - 2)
+    return (2.0 / 3.0) * ni ** 3 + 2.0 * ni ** 2 * nb + 2.0 * nb ** 2 * (ni + nb)
+
+
+def make_problem(n, p, ppw, precision):
+    import torch
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    nwaves = int(n / ppw)
+    kh = (nwaves + 0.03) * 2 * math.pi + 1.8
+    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_lshaped_waveguide(xx, kh))
+    a = 1 / (2 * (n / (p - 2)))
+    dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
+                        precision=precision)
+    uu_dir = lambda xx: bif.uu_dir_func_greens(xx, kh)
+    return dom, uu_dir, kh
+
+
+def run_path(dom, uu_dir, steps, warmup, leaf_events):
+    import torch
+    ts, tf, tsol = [], [], []
+    leaf_ms = []
+    sol = None
+    for it in range(warmup + steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if leaf_events is not None and it >= warmup:
+            dom.hps.leaf_event_pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        else:
+            dom.hps.leaf_event_pair = None
+        info = dom.build(verbose=False)
+        sol, rel, _, _, toc = dom.solve(uu_dir)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if it >= warmup:
+            ts.append(t1 - t0)
+            tf.append(info["toc_build"])
+            tsol.append(toc)
+            if dom.hps.leaf_event_pair is not None:
+                e0, e1 = dom.hps.leaf_event_pair
+                leaf_ms.append(e0.elapsed_time(e1))
+    return ts, tf, tsol, leaf_ms, sol
+
+
+def cpu_baseline(n, p, ppw):
+    """The oracle (numpy/scipy FP64) on a bounded sample of the workload:
+    leaf DtNs of 100 leaves, the slab stage of one slab, two interface LU
+    steps and the interface solve, scaled to the full problem."""
+    import numpy as np
+    import scipy.linalg
+    from oracle import hps_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    nwaves = int(n / ppw)
+    kh = (nwaves + 0.03) * 2 * math.pi + 1.8
+    a = 1 / (2 * (n / (p - 2)))
+    import torch
+    from slablu_amd.src_disc import built_in_funcs as bif
+    bfun = lambda xx, k: bif.bfield_lshaped_waveguide(torch.as_tensor(xx), k).numpy()
+    hps = O.HPSOracle(O.helmholtz_pdo(kh, bfun), np.array([[0, 1.0], [0, 1.0]]), a, p)
+    part = O.SlabPartition(hps.n0, hps.n1, p, 1.0)
+    nleaf = min(100, hps.nboxes)
+    t0 = time.perf_counter()
+    hps.dtns(np.arange(nleaf))
+    t_leaf = (time.perf_counter() - t0) * hps.nboxes / nleaf
+    # one slab of the two-level slab stage (elim LU, rem Schur + LU, dense T_slab)
+    DtN = np.zeros((hps.nboxes, 4 * (p - 2), 4 * (p - 2)))
+    boxes = np.arange(part.buf * hps.n1)
+    DtN[boxes] = hps.dtns(boxes)
+    A = hps.sparse_mat(DtN)
+    sl = O.SlabLUOracle(A, part)
+    t0 = time.perf_counter()
+    sl.build_slab(0)
+    t_slab = (time.perf_counter() - t0) * part.Npan
+    L = hps.n1 * (p - 2)
+    rng = np.random.default_rng(0)
+    S = rng.standard_normal((L, L)) + L * np.eye(L)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        lu = scipy.linalg.lu_factor(S)
+        S2 = S - S @ scipy.linalg.lu_solve(lu, S)
+    t_tred = (time.perf_counter() - t0) / 2 * (part.Npan - 1)
+    t_total = t_leaf + t_slab + t_tred
+    N = O.dof_count(p, hps.n0, hps.n1)
+    return dict(value=N / t_total, unit="DOF/s", cores=cores, kind="port",
+                sample="oracle (numpy/scipy FP64, %d BLAS threads) on %d of %d leaves, 1 of %d slabs, 2 of %d "
+                       "interface LU steps; times scaled to the full %d-DOF problem (leaf %.1f s, slab %.1f s, "
+                       "interface %.1f s)" % (cores, nleaf, hps.nboxes, part.Npan, part.Npan - 1, N, t_leaf,
+                                             t_slab, t_tred))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1000)
+    ap.add_argument("--p", type=int, default=22)
+    ap.add_argument("--ppw", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the SlabLU path on a single MI355X "
+              "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from oracle import hps_oracle as O
+    torch.set_num_threads(len(os.sched_getaffinity(0)))
+    n, p, ppw = args.n, args.p, args.ppw
+    print("bench: mixed path n=%d p=%d" % (n, p), file=sys.stderr, flush=True)
+    dom, uu_dir, kh = make_problem(n, p, ppw, "mixed")
+    N = O.dof_count(p, dom.hps.n0, dom.hps.n1)
+    ts, tf, tsol, leaf_ms, sol_mixed = run_path(dom, uu_dir, args.steps, args.warmup, leaf_events=True)
+    refine_hist = list(dom.refine_history)
+    nboxes = dom.hps.nboxes
+    del dom
+    torch.cuda.empty_cache()
+    fp64 = None
+    if not args.no_fp64:
+        print("bench: fp64 path", file=sys.stderr, flush=True)
+        dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64")
+        ts64, tf64, tsol64, leaf64, sol64 = run_path(dom64, uu_dir, max(1, args.steps), args.warmup,
+                                                      leaf_events=True)
+        err = (torch.linalg.norm(sol_mixed - sol64) / torch.linalg.norm(sol64)).item()
+        fp64 = dict(dof_per_s=N / (sum(ts64) / len(ts64)), ms_per_step=1e3 * sum(ts64) / len(ts64),
+                    factor_ms=1e3 * sum(tf64) / len(tf64), solve_ms=1e3 * sum(tsol64) / len(tsol64),
+                    leaf_ms=sum(leaf64) / max(1, len(leaf64)), err_mixed_vs_fp64=err)
+        del dom64
+        torch.cuda.empty_cache()
+    t_step = sum(ts) / len(ts)
+    leaf_avg = sum(leaf_ms) / len(leaf_ms)
+    flops = nboxes * leaf_flops(p)
+    ach = flops / (leaf_avg * 1e-3) / 1e12
+    roof = dict(bound="mfma", kernel="leaf condensation (FP32): slab_leaf_assemble + slab_getrf + slab_trsm "
+                                     "+ slab_gemm over %d leaves" % nboxes,
+                achieved=ach, peak=PEAK_TFLOPS["f32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["f32"],
+                traffic=None, algorithmic_flops=flops, avg_ms=leaf_avg)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(n, p, ppw)
+    out = {
+        "metric": METRIC, "value": N / t_step, "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "Helmholtz, L-shaped waveguide bumps, unit square, n=%d p=%d ppw=%d kh=%.2f, "
+                               "%dx%d leaves, N=%d, mixed precision + FP64 refinement"
+                               % (n, p, ppw, kh, int(math.sqrt(nboxes)), int(math.sqrt(nboxes)), N),
+                   "N": N, "p": p, "leaves": nboxes, "precision": "mixed"},
+        "factor_ms": 1e3 * sum(tf) / len(tf), "solve_ms": 1e3 * sum(tsol) / len(tsol),
+        "refine_corrections": refine_hist,
+        "fp64": fp64,
+        "speedup_vs_fp64": (fp64["ms_per_step"] / (1e3 * t_step)) if fp64 else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -629,39 +629,51 @@ class SlabLUOracle:
         return I_C, I_elim, I_rem, I_L, I_R
 
     def build(self):
-        P, A = self.part, self.A
-        m_e, bs_e = P.elim_nblocks, P.elim_bs
-        m_r, bs_r = P.rem_nblocks, P.rem_bs
+        P = self.part
         self.elim_fac, self.rem_fac, self.Tslab = [], [], []
         self.elim_blocks, self.rem_blocks = [], []
         for s in range(P.Npan):
-            I_C, I_elim, I_rem, I_L, I_R = self._slab_sets(s)
-            Aee = A[I_elim][:, I_elim].toarray()
-            d, sb, sp = [], [], []
-            for j in range(P.n1):
-                blk = Aee[j * m_e * bs_e:(j + 1) * m_e * bs_e, j * m_e * bs_e:(j + 1) * m_e * bs_e]
-                dd, ss, pp = extract_btri(blk, m_e, bs_e)
-                d.append(dd); sb.append(ss); sp.append(pp)
-            d, sb, sp = np.stack(d), np.stack(sb), np.stack(sp)
-            self.elim_blocks.append((d.copy(), sb.copy(), sp.copy()))
-            efac = btri_factor(d, sb, sp)
-            self.elim_fac.append(efac)
-            # rem Schur complement (exact)
-            Are = A[I_rem][:, I_elim].toarray()
-            Aer = A[I_elim][:, I_rem].toarray()
-            Arr = A[I_rem][:, I_rem].toarray()
-            Y = btri_solve(efac, Aer.reshape(P.n1, m_e * bs_e, -1)).reshape(len(I_elim), -1)
-            Sr = Arr - Are @ Y
-            dr, sbr, spr = extract_btri(Sr, m_r, bs_r)
-            self.rem_blocks.append((dr.copy(), sbr.copy(), spr.copy()))
-            self.rem_fac.append(btri_factor(dr[None], sbr[None], spr[None]))
-            # slab Schur complement on X = [L; R]
-            I_X = np.hstack((I_L, I_R))
-            ACX = A[I_C][:, I_X].toarray()
-            V = self.slab_solve(s, ACX)
-            AXC = A[I_X][:, I_C].toarray()
-            self.Tslab.append(-(AXC @ V))
-        # interface system (block_tridiag_solver_structured.py:34-76)
+            self.build_slab(s)
+        return self.build_interface()
+
+    def build_slab(self, s):
+        """Elim LU, rem Schur complement + LU and dense T_slab of slab s."""
+        P, A = self.part, self.A
+        m_e, bs_e = P.elim_nblocks, P.elim_bs
+        m_r, bs_r = P.rem_nblocks, P.rem_bs
+        if not hasattr(self, "elim_fac"):
+            self.elim_fac, self.rem_fac, self.Tslab = [], [], []
+            self.elim_blocks, self.rem_blocks = [], []
+        I_C, I_elim, I_rem, I_L, I_R = self._slab_sets(s)
+        Aee = A[I_elim][:, I_elim].toarray()
+        d, sb, sp = [], [], []
+        for j in range(P.n1):
+            blk = Aee[j * m_e * bs_e:(j + 1) * m_e * bs_e, j * m_e * bs_e:(j + 1) * m_e * bs_e]
+            dd, ss, pp = extract_btri(blk, m_e, bs_e)
+            d.append(dd); sb.append(ss); sp.append(pp)
+        d, sb, sp = np.stack(d), np.stack(sb), np.stack(sp)
+        self.elim_blocks.append((d.copy(), sb.copy(), sp.copy()))
+        efac = btri_factor(d, sb, sp)
+        self.elim_fac.append(efac)
+        # rem Schur complement (exact)
+        Are = A[I_rem][:, I_elim].toarray()
+        Aer = A[I_elim][:, I_rem].toarray()
+        Arr = A[I_rem][:, I_rem].toarray()
+        Y = btri_solve(efac, Aer.reshape(P.n1, m_e * bs_e, -1)).reshape(len(I_elim), -1)
+        Sr = Arr - Are @ Y
+        dr, sbr, spr = extract_btri(Sr, m_r, bs_r)
+        self.rem_blocks.append((dr.copy(), sbr.copy(), spr.copy()))
+        self.rem_fac.append(btri_factor(dr[None], sbr[None], spr[None]))
+        # slab Schur complement on X = [L; R]
+        I_X = np.hstack((I_L, I_R))
+        ACX = A[I_C][:, I_X].toarray()
+        V = self.slab_solve(s, ACX)
+        AXC = A[I_X][:, I_C].toarray()
+        self.Tslab.append(-(AXC @ V))
+
+    def build_interface(self):
+        """Interface system (block_tridiag_solver_structured.py:34-76)."""
+        P, A = self.part, self.A
         L = P.n1 * P.f
         nI = P.Npan - 1
         self.T_diag = np.zeros((nI, L, L))

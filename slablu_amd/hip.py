@@ -2,9 +2,9 @@
 
 Every function takes torch tensors that live on the GPU and passes their
 data pointers and strides to the C ABI; all launches are asynchronous on the
-current torch stream.  The library is mandatory: if it is missing, importing
-this module raises, and there is no CPU fallback anywhere in the product
-path.
+current torch stream.  The library is mandatory: it is loaded on first use
+and a missing library raises there; there is no CPU fallback anywhere in
+the product path.
 """
 import ctypes
 import os
@@ -49,7 +49,16 @@ def _load():
     return lib
 
 
-LIB = _load()
+class _Lazy:
+    _lib = None
+
+    def __getattr__(self, name):
+        if _Lazy._lib is None:
+            _Lazy._lib = _load()
+        return getattr(_Lazy._lib, name)
+
+
+LIB = _Lazy()
 
 
 def _check(rc, name):

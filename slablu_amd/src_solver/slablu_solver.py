@@ -1,0 +1,564 @@
+"""SlabLU two-level direct solver for HPS discretizations, on the GPU
+(drop-in for src_solver/slablu_solver.py of the reference).
+
+Domain_Solver(box_geom, pdo_op, kh, h, p, buf_constant, periodic_bc)
+  .build(sparse_assembly, solver_type, verbose) -> info dict
+  .solve(uu_dir_func, ff_body_func=None, known_sol=False)
+        -> (sol_tot, rel_err, true_err, resloc_hps, toc_solve)
+
+Factor (everything batched on the device; FP64 by default):
+  1. leaf DtN maps of all leaves                   (hps_parallel_leaf_ops)
+  2. Aloc: per leaf row of every slab, the block-tridiagonal system of the
+     buf-1 interior vertical faces, gathered directly from the DtN blocks
+     (the reference recovers the same blocks by probing A,
+     slablu_solver.py:718-745) and LU-factored (Block_Tridiagonal).
+  3. Ared: Schur complement on the interior horizontal faces of each slab,
+     block tridiagonal over the n1-1 face rows (slablu_solver.py:768-882),
+     formed exactly as A_rr - sum_j A_r,e_j A_e_j^{-1} A_e_j,r and factored.
+  4. Slab Schur complements on [L; R] (the operator get_TXX_slabind samples,
+     slablu_solver.py:1066-1211) formed densely with multi-RHS sweeps, and
+     the interface block-tridiagonal system over the Npan-1 slab interfaces
+     (slablu_solver.py:1215-1308) factored in FP64 (HBS_BlockTridiagonal).
+Solve: right-hand side from the Dirichlet data, slab sweeps, interface
+sweep, slab sweeps, leaf recovery (slablu_solver.py:489-666).
+
+precision='mixed' runs steps 1-4a in FP32 (interface LU in FP64) and
+refines the full discrete system (leaf collocation equations + flux
+continuity on the faces) with FP64 residuals until the correction falls
+below `refine_tol`.
+"""
+import time
+from functools import reduce
+
+import numpy as np
+import torch
+
+from .. import hip
+from ..src_disc import hps_multidomain_disc
+from .block_tridiag_solver_dense import Block_Tridiagonal
+from .block_tridiag_solver_structured import HBS_BlockTridiagonal
+
+L_, R_, D_, U_ = 0, 1, 2, 3
+
+
+def get_nearest_div(n, x):
+    """Divisor of n nearest to x, ties broken towards the larger divisor
+    (slablu_solver.py:35-46)."""
+    factors = list(reduce(list.__add__, ([i, n // i] for i in range(1, int(n ** 0.5) + 1) if n % i == 0)))
+    best, dist = n, abs(n - x)
+    for f in factors:
+        d = abs(f - x)
+        if d < dist or (d == dist and f > best):
+            best, dist = f, d
+    return int(best)
+
+
+def _blockdiag(blocks):
+    """(..., k, f, f) -> (..., k*f, k*f) block diagonal."""
+    *lead, k, f, _ = blocks.shape
+    out = torch.zeros(*lead, k, f, k, f, dtype=blocks.dtype, device=blocks.device)
+    idx = torch.arange(k, device=blocks.device)
+    nl = len(lead)
+    perm = (nl,) + tuple(range(nl)) + (nl + 1, nl + 2)
+    out[(slice(None),) * nl + (idx, slice(None), idx)] = blocks.permute(*perm)
+    return out.reshape(*lead, k * f, k * f)
+
+
+class Domain_Solver:
+    def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
+                 device=None, refine_tol=1e-11, refine_maxit=8):
+        if p == 0:
+            raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
+        if periodic_bc:
+            raise NotImplementedError("periodic boundary conditions are not provided")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.kh = kh
+        self.periodic_bc = periodic_bc
+        self.buf_constant = buf_constant
+        self.precision = precision
+        self.refine_tol, self.refine_maxit = refine_tol, refine_maxit
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        self.hps_disc(box_geom, h, p, pdo_op)
+        self.hps_panel_split()
+        self.disc = "hps"
+        self.ntot = self.hps.xx_active.shape[0]
+        I_L, I_C, Npan = self.I_L, self.I_C, self.Npan
+        self.I_slabC = self.inds_pans[:, I_C].flatten()
+        self.I_slabX = self.inds_pans[1:, I_L].flatten()
+        self.I_Ctot = torch.hstack((self.I_slabC, self.I_slabX))
+        I_Ldir = self.inds_pans[0, I_L]
+        I_Rdir = self.inds_pans[Npan - 1, self.I_R]
+        I_Ddir = self.inds_pans[:, self.I_D].flatten()
+        I_Udir = self.inds_pans[:, self.I_U].flatten()
+        self.I_Xtot = torch.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+
+    # ------------------------------------------------------------------ setup
+    def hps_disc(self, box_geom, a, p, pdo_op):
+        """Leaf grid and slab width (slablu_solver.py:127-163)."""
+        H = hps_multidomain_disc.HPS_Multidomain(pdo_op, box_geom, a, p, device=self.device)
+        f = p - 2
+        n0, n1 = H.n0, H.n1
+        n_tmp = max(n0, n1) * f - 1
+        buf_points = min(400, int(n_tmp ** (2 / 3) * self.buf_constant))
+        buf = get_nearest_div(n0, max(int(buf_points / f) + 1, 2))
+        if buf < 2:
+            raise ValueError("each slab needs at least 2 leaf columns (n0=%d)" % n0)
+        Npan = n0 // buf
+        nfaces_pan = (2 * n1 + 1) * buf + n1
+        self.inds_pans = torch.stack([torch.arange(nfaces_pan * f) + (2 * n1 + 1) * buf * j * f
+                                      for j in range(Npan)])
+        self.Npan, self.Npan_loc, self.buf_pans = Npan, n1, buf
+        self.elim_nblocks, self.elim_bs = buf - 1, f
+        self.rem_nblocks, self.rem_bs = n1 - 1, buf * f
+        self.hps = H
+
+    def hps_panel_split(self):
+        """Slab-local index sets I_L, I_R, I_D, I_U, I_C = [I_elim, I_rem]
+        (slablu_solver.py:165-202)."""
+        f = self.hps.p - 2
+        n1, buf = self.Npan_loc, self.buf_pans
+        self.I_L = torch.arange(n1 * f)
+        self.I_R = torch.arange(n1 * f) + (2 * n1 + 1) * buf * f
+        bo = (2 * n1 + 1) * torch.arange(buf)
+        t = torch.arange(f)
+        self.I_D = ((bo + n1)[:, None] * f + t).flatten()
+        self.I_U = ((bo + 2 * n1)[:, None] * f + t).flatten()
+        jj = torch.arange(1, n1)
+        I_rem = ((bo[None, :, None] + n1 + jj[:, None, None]) * f + t[None, None, :])
+        I_elim = ((bo[None, 1:, None] + torch.arange(n1)[:, None, None]) * f + t[None, None, :])
+        self.I_C = torch.hstack((I_elim.flatten(), I_rem.flatten()))
+
+    # ------------------------------------------------------------------ build
+    def build(self, sparse_assembly="reduced_gpu", solver_type="slabLU", verbose=True):
+        if solver_type not in (None, "slabLU"):
+            raise NotImplementedError("solver_type %r: only 'slabLU' is provided" % (solver_type,))
+        self.sparse_assembly = sparse_assembly
+        self.solver_type = "slabLU"
+        self.wdt = torch.float32 if self.precision == "mixed" else torch.float64
+        t = {}
+        sync = torch.cuda.synchronize
+        sync()
+        tic0 = time.time()
+        self.hps.factor_leaves(self.device, self.wdt)
+        sync()
+        t["toc_DtN"] = time.time() - tic0
+        self.Dt7 = self._dt7(self.hps.leaf["DtN"])
+        tic = time.time()
+        self.build_Aloc_solver()
+        sync()
+        t["toc_Aloc"] = time.time() - tic
+        tic = time.time()
+        self.construct_Ared()
+        sync()
+        t["toc_Ared"] = time.time() - tic
+        tic = time.time()
+        self.build_Tred_solver()
+        sync()
+        t["toc_Tred"] = time.time() - tic
+        toc_build = time.time() - tic0
+        mem = (self.hps.leaf["M"].numel() * self.hps.leaf["M"].element_size() + self.Aloc_solver.nbytes +
+               self.Ared_solver.nbytes + self.Tred_solver.nbytes) / 1e9
+        info = dict(toc_build=toc_build, mem_build=mem, toc_assembly=t["toc_DtN"], **t)
+        if verbose:
+            print("SLABLU BUILD (%s): DtN %.4f s, Aloc %.4f s, Ared %.4f s, Tred %.4f s, total %.4f s, %.2f GB"
+                  % (self.precision, t["toc_DtN"], t["toc_Aloc"], t["toc_Ared"], t["toc_Tred"], toc_build, mem))
+        self.build_info = info
+        return info
+
+    def _dt7(self, DtN):
+        """DtN blocks indexed [slab, slab column, leaf row, a, :, b, :] with
+        a, b in (l, r, d, u)."""
+        f = self.hps.p - 2
+        return DtN.view(self.Npan, self.buf_pans, self.Npan_loc, 4, f, 4, f)
+
+    def _blk(self, c, a, b):
+        """(Npan, |c|, n1, f, f) blocks DtN[a, b] of the leaves in slab column(s) c."""
+        return self.Dt7[:, c, :, a, :, b, :]
+
+    def elim_blocks(self):
+        """Per leaf row, the chain of buf-1 vertical faces: diag_q =
+        DtN_q[r,r] + DtN_{q+1}[l,l], sup_q = DtN_{q+1}[l,r], sub_q = DtN_{q+1}[r,l]
+        -> (Npan*n1, m, f, f), (Npan*n1, m-1, f, f) x 2."""
+        Np, n1, buf, f = self.Npan, self.Npan_loc, self.buf_pans, self.hps.p - 2
+        perm = lambda x: x.permute(0, 2, 1, 3, 4).reshape(Np * n1, -1, f, f)
+        diag = perm(self._blk(slice(0, buf - 1), R_, R_) + self._blk(slice(1, buf), L_, L_))
+        sup = perm(self._blk(slice(1, buf - 1), L_, R_))
+        sub = perm(self._blk(slice(1, buf - 1), R_, L_))
+        return diag, sub, sup
+
+    def rem_blocks(self):
+        """A_rr of each slab: block tridiagonal over the n1-1 interior face
+        rows, each block block-diagonal over the slab columns ->
+        diag (Npan, n1-1, B, B), sup/sub (Npan, n1-2, B, B)."""
+        n1 = self.Npan_loc
+        mr = n1 - 1
+        rows, nxt = slice(0, n1 - 1), slice(1, n1)
+        bd = lambda x: _blockdiag(x.permute(0, 2, 1, 3, 4))
+        d = bd(self.Dt7[:, :, rows, U_, :, U_, :] + self.Dt7[:, :, nxt, D_, :, D_, :])
+        u = bd(self.Dt7[:, :, nxt, D_, :, U_, :])[:, :mr - 1]
+        l = bd(self.Dt7[:, :, nxt, U_, :, D_, :])[:, :mr - 1]
+        return d, u, l
+
+    def interface_blocks(self):
+        """A restricted to each slab interface (block diagonal over the leaf
+        rows): (Npan-1, n1*f, n1*f)."""
+        Np, buf = self.Npan, self.buf_pans
+        return _blockdiag(self.Dt7[:Np - 1, buf - 1, :, R_, :, R_, :] + self.Dt7[1:, 0, :, L_, :, L_, :])
+
+    def build_Aloc_solver(self):
+        Np, n1, f = self.Npan, self.Npan_loc, self.hps.p - 2
+        diag, sub, sup = self.elim_blocks()
+        self.Aloc_solver = Block_Tridiagonal(Np * n1, self.buf_pans - 1, f)
+        self.Aloc_solver.build_sweep(diag, sub, sup)
+
+    def _couplings(self):
+        """Dense per-leaf-row coupling blocks between the elim chain of leaf
+        row j, the local rem faces [bottom faces; top faces] (2*buf*f) and the
+        slab interfaces [L_j; R_j] (2f), for all (slab, row)."""
+        Np, n1, buf, f = self.Npan, self.Npan_loc, self.buf_pans, self.hps.p - 2
+        m, B = buf - 1, buf * f
+        dt, dev = self.wdt, self.device
+        q = torch.arange(m, device=dev)
+        # R = A[elim, rem_local]: (Np, n1, m, f, 2, buf, f)
+        Rm = torch.zeros(Np, n1, m, f, 2, buf, f, dtype=dt, device=dev)
+        Cm = torch.zeros(Np, n1, 2, buf, f, m, f, dtype=dt, device=dev)
+        for s_, side in ((0, D_), (1, U_)):
+            # l face of leaf q+1 is elim face q; r face of leaf q is elim face q
+            Rm[:, :, q, :, s_, q + 1, :] = self._blk(slice(1, buf), L_, side).permute(1, 0, 2, 3, 4)
+            Rm[:, :, q, :, s_, q, :] = self._blk(slice(0, buf - 1), R_, side).permute(1, 0, 2, 3, 4)
+            Cm[:, :, s_, q + 1, :, q, :] = self._blk(slice(1, buf), side, L_).permute(1, 0, 2, 3, 4)
+            Cm[:, :, s_, q, :, q, :] = self._blk(slice(0, buf - 1), side, R_).permute(1, 0, 2, 3, 4)
+        # E = A[elim, X_j], AXe = A[X_j, elim]
+        Em = torch.zeros(Np, n1, m, f, 2, f, dtype=dt, device=dev)
+        AXe = torch.zeros(Np, n1, 2, f, m, f, dtype=dt, device=dev)
+        Em[:, :, 0, :, 0, :] = self._blk(0, R_, L_)
+        Em[:, :, m - 1, :, 1, :] = self._blk(buf - 1, L_, R_)
+        AXe[:, :, 0, :, 0, :] = self._blk(0, L_, R_)
+        AXe[:, :, 1, :, m - 1, :] = self._blk(buf - 1, R_, L_)
+        # Xr = A[rem_local, X_j], AXr = A[X_j, rem_local]
+        Xr = torch.zeros(Np, n1, 2, buf, f, 2, f, dtype=dt, device=dev)
+        AXr = torch.zeros(Np, n1, 2, f, 2, buf, f, dtype=dt, device=dev)
+        for s_, side in ((0, D_), (1, U_)):
+            Xr[:, :, s_, 0, :, 0, :] = self._blk(0, side, L_)
+            Xr[:, :, s_, buf - 1, :, 1, :] = self._blk(buf - 1, side, R_)
+            AXr[:, :, 0, :, s_, 0, :] = self._blk(0, L_, side)
+            AXr[:, :, 1, :, s_, buf - 1, :] = self._blk(buf - 1, R_, side)
+        nb = Np * n1
+        self.cR = Rm.reshape(nb, m * f, 2 * B)
+        self.cC = Cm.reshape(nb, 2 * B, m * f)
+        self.cE = Em.reshape(nb, m * f, 2 * f)
+        self.cAXe = AXe.reshape(nb, 2 * f, m * f)
+        self.cXr = Xr.reshape(nb, 2 * B, 2 * f)
+        self.cAXr = AXr.reshape(nb, 2 * f, 2 * B)
+
+    def construct_Ared(self):
+        """Schur complement of each slab onto its interior horizontal faces,
+        block tridiagonal over the n1-1 face rows (blocks buf*f)."""
+        Np, n1, buf, f = self.Npan, self.Npan_loc, self.buf_pans, self.hps.p - 2
+        m, B = buf - 1, buf * f
+        self._couplings()
+        nb = Np * n1
+        # [W | Y] = A_ee^{-1} [R | E]
+        RE = torch.cat((self.cR, self.cE), dim=2).contiguous()
+        self.Aloc_solver.lusolve_sweep(RE)
+        self.cW = RE[:, :, :2 * B]
+        self.cY = RE[:, :, 2 * B:]
+        CWY = torch.empty(nb, 2 * B, 2 * B + 2 * f, dtype=self.wdt, device=self.device)
+        hip.gemm(self.cC, RE, CWY)
+        CW = CWY[:, :, :2 * B].reshape(Np, n1, 2, B, 2, B)
+        self.cCY = CWY[:, :, 2 * B:]
+        mr = n1 - 1
+        Arr_d, Arr_u, Arr_l = self.rem_blocks()
+        diag = Arr_d - CW[:, :mr, 1, :, 1, :] - CW[:, 1:, 0, :, 0, :]
+        sup = Arr_u - CW[:, 1:mr, 0, :, 1, :]
+        sub = Arr_l - CW[:, 1:mr, 1, :, 0, :]
+        self.Ared_solver = Block_Tridiagonal(Np, mr, B)
+        self.Ared_solver.build_sweep(diag.contiguous(), sub.contiguous(), sup.contiguous())
+
+    def _slab_u_local(self, upad, s):
+        """(n1, 2B, ncols) view of the padded rem solution of slab s: rows of
+        leaf row j are [bottom faces; top faces] = padded blocks j, j+1."""
+        n1, B = self.Npan_loc, self.rem_bs
+        nc = upad.shape[-1]
+        return torch.as_strided(upad, (n1, 2 * B, nc), (B * nc, nc, 1),
+                                upad.storage_offset() + s * upad.stride(0))
+
+    def build_Tred_solver(self):
+        """Dense slab Schur complements on [L; R] and the interface system."""
+        Np, n1, buf, f = self.Npan, self.Npan_loc, self.buf_pans, self.hps.p - 2
+        B, L = self.rem_bs, n1 * f
+        nb = Np * n1
+        dt, dev = self.wdt, self.device
+        if Np < 2:
+            self.Tred_solver = None
+            return
+        # G = A[rem, X_j] - A[rem, e_j] Y_j, scattered into the padded rem rows
+        G = self.cXr - self.cCY
+        upad = torch.zeros(Np, (n1 + 1) * B, 2 * L, dtype=dt, device=dev)
+        g6 = upad.view(Np, n1 + 1, B, 2, n1, f)
+        G6 = G.reshape(Np, n1, 2, B, 2, f)
+        ja = torch.arange(n1, device=dev)
+        for sg in (0, 1):
+            g6[:, ja + sg, :, :, ja, :] = G6[:, :, sg].permute(1, 0, 2, 3, 4)
+        u = upad[:, B:n1 * B]
+        self.Ared_solver.lusolve_sweep(u)
+        # the padding blocks stand for the Dirichlet rows D and U: zero them
+        upad[:, :B] = 0
+        upad[:, n1 * B:] = 0
+        # K_j = (A[X_j, rem] - A[X_j, e_j] W_j) u_local_j + A[X_j, e_j] Y_j
+        H = self.cAXr.clone()
+        hip.gemm(self.cAXe, self.cW, H, alpha=-1.0, beta=1.0)
+        self.cH = H
+        K = torch.empty(Np, n1, 2 * f, 2 * L, dtype=dt, device=dev)
+        for s in range(Np):
+            hip.gemm(H[s * n1:(s + 1) * n1], self._slab_u_local(upad, s), K[s])
+        Q = torch.empty(nb, 2 * f, 2 * f, dtype=dt, device=dev)
+        hip.gemm(self.cAXe, self.cY, Q)
+        K6 = K.view(Np, n1, 2 * f, 2, n1, f)
+        K6[:, ja, :, :, ja, :] += Q.view(Np, n1, 2 * f, 2, f).permute(1, 0, 2, 3, 4)
+        Kf = K.view(Np, n1, 2, f, 2 * L).permute(0, 2, 1, 3, 4).reshape(Np, 2 * L, 2 * L)
+        del upad, K
+        # interface s between slab s and s+1
+        Aii = self.interface_blocks()
+        KR = Kf[:, L:, L:].to(torch.float64)
+        KL = Kf[:, :L, :L].to(torch.float64)
+        diag = Aii.to(torch.float64) - KR[:Np - 1] - KL[1:]
+        sup = -Kf[1:Np - 1, :L, L:].to(torch.float64)
+        sub = -Kf[1:Np - 1, L:, :L].to(torch.float64)
+        self.Tred_solver = HBS_BlockTridiagonal(Np - 1, L)
+        self.Tred_solver.build_lowmem(diag, None, sub, sup, device=dev, dtype=torch.float64)
+
+    # ------------------------------------------------------------------ solve
+    def _upad(self, rem, ncols):
+        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        upad = torch.zeros(Np, (n1 + 1) * B, ncols, dtype=rem.dtype, device=rem.device)
+        upad[:, B:n1 * B] = rem
+        return upad
+
+    def _u_local_all(self, upad):
+        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        nc = upad.shape[-1]
+        return torch.as_strided(upad, (Np, n1, 2 * B, nc), (upad.stride(0), B * nc, nc, 1),
+                                upad.storage_offset()).reshape(Np * n1, 2 * B, nc)
+
+    def _scatter_rem(self, CT):
+        """Sum per-leaf-row [bottom; top] rem contributions (Np*n1, 2B, nc)
+        into the rem rows of each slab -> (Np, (n1-1)*B, nc)."""
+        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        nc = CT.shape[-1]
+        C5 = CT.view(Np, n1, 2, B, nc)
+        g = torch.zeros(Np, n1 + 1, B, nc, dtype=CT.dtype, device=CT.device)
+        g[:, :n1] += C5[:, :, 0]
+        g[:, 1:] += C5[:, :, 1]
+        return g[:, 1:n1].reshape(Np, (n1 - 1) * B, nc)
+
+    def solve_slab(self, f_C, transpose=False):
+        """A_CC(slab)^{-1} f for every slab; f_C (Npan, |I_C|, nrhs)."""
+        if transpose:
+            raise NotImplementedError("transposed slab solves are not provided")
+        Np, n1, m, f = self.Npan, self.Npan_loc, self.elim_nblocks, self.elim_bs
+        ne = n1 * m * f
+        nc = f_C.shape[-1]
+        fe = f_C[:, :ne].reshape(Np * n1, m * f, nc).contiguous()
+        fr = f_C[:, ne:]
+        t = self.Aloc_solver.lusolve_sweep(fe.clone())
+        CT = torch.empty(Np * n1, 2 * self.rem_bs, nc, dtype=t.dtype, device=t.device)
+        hip.gemm(self.cC, t, CT)
+        g = fr - self._scatter_rem(CT)
+        upad = self._upad(g, nc)
+        self.Ared_solver.lusolve_sweep(upad[:, self.rem_bs:n1 * self.rem_bs])
+        ue = t.clone()
+        hip.gemm(self.cW, self._u_local_all(upad), ue, alpha=-1.0, beta=1.0)
+        sol = torch.empty_like(f_C)
+        sol[:, :ne] = ue.view(Np, ne, nc)
+        sol[:, ne:] = upad[:, self.rem_bs:n1 * self.rem_bs]
+        return sol, ue, upad
+
+    def _AXC(self, ue, upad):
+        """Z_j = A[X_j, C] y for every (slab, leaf row): (Np, n1, 2, f, nc)."""
+        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+        nc = ue.shape[-1]
+        Z = torch.empty(Np * n1, 2 * f, nc, dtype=ue.dtype, device=ue.device)
+        hip.gemm(self.cAXe, ue, Z)
+        hip.gemm(self.cAXr, self._u_local_all(upad), Z, beta=1.0)
+        return Z.view(Np, n1, 2, f, nc)
+
+    def _ACX(self, x):
+        """A[C, X] x for every slab from x (Np, n1, 2f, nc) -> (Np, |I_C|, nc)."""
+        Np, n1, m, f = self.Npan, self.Npan_loc, self.elim_nblocks, self.elim_bs
+        nc = x.shape[-1]
+        xe = x.reshape(Np * n1, 2 * f, nc).contiguous()
+        e = torch.empty(Np * n1, m * f, nc, dtype=x.dtype, device=x.device)
+        hip.gemm(self.cE, xe, e)
+        r = torch.empty(Np * n1, 2 * self.rem_bs, nc, dtype=x.dtype, device=x.device)
+        hip.gemm(self.cXr, xe, r)
+        out = torch.empty(Np, len(self.I_C), nc, dtype=x.dtype, device=x.device)
+        out[:, :n1 * m * f] = e.view(Np, n1 * m * f, nc)
+        out[:, n1 * m * f:] = self._scatter_rem(r)
+        return out
+
+    def solve_reduced(self, ff):
+        """A_CC x = ff on the face unknowns I_Ctot (slablu_solver.py:543-590);
+        ff (|I_Ctot|, nrhs) in the working precision (interface in FP64)."""
+        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+        nC = len(self.I_C)
+        L = n1 * f
+        nc = ff.shape[-1]
+        ffC = ff[:Np * nC].view(Np, nC, nc)
+        yC, ue, upad = self.solve_slab(ffC)
+        if Np == 1:
+            return yC.reshape(Np * nC, nc)
+        Z = self._AXC(ue, upad)
+        ffX = ff[Np * nC:].view(Np - 1, n1, f, nc).to(torch.float64) - Z[:Np - 1, :, 1].double() \
+            - Z[1:, :, 0].double()
+        solX = self.Tred_solver.solve(ffX.reshape((Np - 1) * L, nc))
+        sx = solX.view(Np - 1, n1, f, nc)
+        x = torch.zeros(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
+        x[1:, :, 0] = sx.to(ff.dtype)
+        x[:Np - 1, :, 1] = sx.to(ff.dtype)
+        solC, _, _ = self.solve_slab(ffC - self._ACX(x.view(Np, n1, 2 * f, nc)))
+        return torch.cat((solC.reshape(Np * nC, nc), solX.to(ff.dtype)), dim=0)
+
+    def get_rhs(self, uu_dir_func, ff_body_func=None):
+        """ff = -A[Ctot, Xtot] u_dir (slablu_solver.py:489-526) on the device
+        (FP64), plus the Dirichlet values."""
+        xx = self.hps.xx_active[self.I_Xtot]
+        uu_dir = torch.as_tensor(uu_dir_func(xx)).to(self.device, torch.float64)
+        if uu_dir.dim() == 1:
+            uu_dir = uu_dir[:, None]
+        x = torch.zeros(self.hps.nface, uu_dir.shape[1], dtype=torch.float64, device=self.device)
+        x[self.I_Xtot.to(self.device)] = uu_dir
+        return x, uu_dir
+
+    def _face_residual(self, x_face, DtN):
+        Ic = self.I_Ctot.to(self.device)
+        return self.hps.apply_A(x_face, DtN)[Ic]
+
+    def solve_helper(self, uu_dir_func, ff_body_func=None):
+        """Two-level solve; returns (face solution with Dirichlet values,
+        relative residual of the reduced system, time)."""
+        dev = self.device
+        Ic = self.I_Ctot.to(dev)
+        torch.cuda.synchronize()
+        tic = time.time()
+        x_face, uu_dir = self.get_rhs(uu_dir_func)
+        DtN = self.hps.leaf["DtN"]
+        ff = -self._face_residual(x_face, DtN)
+        if ff_body_func is not None:
+            fb = self.hps.body_load(ff_body_func, ff.shape[-1], DtN.dtype)
+            ff = ff + self._body_reduce(fb)[Ic]
+        sol = self.solve_reduced(ff.to(self.wdt))
+        x_face[Ic] = sol.to(torch.float64)
+        torch.cuda.synchronize()
+        toc = time.time() - tic
+        xc = torch.zeros_like(x_face)
+        xc[Ic] = x_face[Ic]
+        res = self._face_residual(xc, DtN).double() - ff.double()
+        rel = (torch.linalg.norm(res) / torch.linalg.norm(ff.double())).item()
+        return x_face, rel, toc
+
+    def _body_reduce(self, fb):
+        """Face load -sum_b P_b N_c A_cc^{-1} f_b of interior loads fb
+        (nboxes, ni, nrhs) (the reduction reduce_body intends,
+        hps_parallel_leaf_ops.py:94-100)."""
+        ops, lf = self.hps.ops, self.hps.leaf
+        ni = ops.ni
+        w = fb.to(lf["dtype"]).contiguous().clone()
+        hip.getrs(lf["M"][:, :, :ni], lf["ipiv"], w)
+        z = torch.empty(fb.shape[0], ops.nx, fb.shape[-1], dtype=w.dtype, device=w.device)
+        hip.gemm(ops.typed(w.dtype)["Nx_c"], w, z, alpha=-1.0)
+        y = torch.zeros(self.hps.nface, fb.shape[-1], dtype=w.dtype, device=w.device)
+        y.index_add_(0, self.hps.Iext_d.reshape(-1), z.reshape(-1, fb.shape[-1]))
+        return y
+
+    def solve(self, uu_dir_func, ff_body_func=None, known_sol=False):
+        """Returns (sol_tot at all leaf nodes (cpu), rel_err, true_err,
+        resloc_hps, toc_solve) as slablu_solver.py:623-666 does."""
+        if self.precision == "mixed":
+            uu, rel, toc, resloc = self.solve_refined(uu_dir_func, ff_body_func)
+            self.sol_dev = uu
+        else:
+            x_face, rel, toc = self.solve_helper(uu_dir_func, ff_body_func)
+            torch.cuda.synchronize()
+            tic = time.time()
+            uu, resloc = self.hps.solve(self.device, x_face, ff_body_func)
+            torch.cuda.synchronize()
+            toc += time.time() - tic
+            self.sol_dev = uu
+        true_err = float("nan")
+        if known_sol:
+            uu_true = torch.as_tensor(uu_dir_func(self.hps.xx_tot.clone())).to(self.device, torch.float64)
+            true_err = (torch.linalg.norm(uu.double() - uu_true) / torch.linalg.norm(uu_true)).item()
+        return uu.double().cpu(), rel, true_err, resloc, toc
+
+    # ------------------------------------------------------- mixed precision
+    def full_residual(self, x_face, uu):
+        """FP64 residual of the un-condensed system at the current iterate:
+        r_I = f - A_loc u at leaf interiors, r_F = -(sum of outward fluxes) at
+        the non-Dirichlet face DOFs."""
+        hps = self.hps
+        ops = hps.ops
+        Au = ops.apply_aloc(uu, hps.coefs)
+        rI = -Au[:, ops.Jc]
+        fl = ops.flux(uu)
+        y = torch.zeros(hps.nface, uu.shape[-1], dtype=torch.float64, device=self.device)
+        y.index_add_(0, hps.Iext_d.reshape(-1), fl.reshape(-1, uu.shape[-1]))
+        return rI, -y[self.I_Ctot.to(self.device)]
+
+    def solve_refined(self, uu_dir_func, ff_body_func=None):
+        """Mixed precision: FP32 factors, FP64 residuals of the full system,
+        corrections until ||delta|| <= refine_tol ||u||."""
+        if ff_body_func is not None:
+            raise NotImplementedError("body loads with precision='mixed' are not provided")
+        hps, ops = self.hps, self.hps.ops
+        dev = self.device
+        Ic = self.I_Ctot.to(dev)
+        lf = hps.leaf
+        torch.cuda.synchronize()
+        tic = time.time()
+        x_face, uu_dir = self.get_rhs(uu_dir_func)
+        nc = uu_dir.shape[1]
+        ub = hps.faces_to_boxes(x_face)
+        uu = ops.assemble_nodes(ub, torch.zeros(hps.nboxes, ops.ni, nc, dtype=torch.float64, device=dev))
+        self.refine_history = []
+        for it in range(self.refine_maxit):
+            rI, rF = self.full_residual(x_face, uu)
+            # correction: A dF = rF - sum_b P_b Nc Acc^{-1} rI ; dI = Acc^{-1} rI - S' dF
+            ff = rF + self._body_reduce(rI)[Ic].double()
+            dC = self.solve_reduced(ff.to(self.wdt))
+            dF = torch.zeros_like(x_face)
+            dF[Ic] = dC.double()
+            db = hps.faces_to_boxes(dF).to(self.wdt).contiguous()
+            dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt))
+            x_face += dF
+            uu[:, ops.Jc] += dI.double()
+            ub = hps.faces_to_boxes(x_face)
+            ureo = torch.empty(hps.nboxes, ops.Intmap.shape[0], nc, dtype=torch.float64, device=dev)
+            hip.gemm(ops.Intmap, ub, ureo)
+            uu[:, ops.Jxreo] = ureo
+            rel_corr = (torch.linalg.norm(dF) / torch.linalg.norm(x_face)).item()
+            self.refine_history.append(rel_corr)
+            if rel_corr <= self.refine_tol:
+                break
+        torch.cuda.synchronize()
+        toc = time.time() - tic
+        rI, rF = self.full_residual(x_face, uu)
+        resloc = torch.linalg.norm(rI).item()
+        ff0 = -self._face_residual(self.get_rhs(uu_dir_func)[0], lf["DtN"]).double()
+        rel = (torch.linalg.norm(rF) / torch.linalg.norm(ff0)).item()
+        self.last_face = x_face
+        return uu.reshape(-1, nc), rel, toc, resloc
+
+    # ------------------------------------------------------- self checks
+    def test_slab_solve(self, nrhs=3):
+        """||A_CC(slab) x - f|| / ||f|| for random f, all slabs at once
+        (slablu_solver.py:945-959), using the matrix-free face operator."""
+        Np, nC = self.Npan, len(self.I_C)
+        f = torch.rand(Np, nC, nrhs, dtype=self.wdt, device=self.device)
+        sol, _, _ = self.solve_slab(f)
+        xf = torch.zeros(self.hps.nface, nrhs, dtype=torch.float64, device=self.device)
+        IsC = self.I_slabC.to(self.device)
+        xf[IsC] = sol.reshape(-1, nrhs).double()
+        y = self.hps.apply_A(xf, self.hps.leaf["DtN"].double())[IsC]
+        return (torch.linalg.norm(y - f.reshape(-1, nrhs).double()) / torch.linalg.norm(f.double())).item()

@@ -1,0 +1,132 @@
+"""End-to-end parity of the GPU SlabLU path (Domain_Solver) with the CPU
+oracle (exact discrete solution through SuperLU) and with the reference's
+own figures (tests/golden/reference_figures.json)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hps_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _fig(tag):
+    with open(os.path.join(GOLD, "reference_figures.json")) as fh:
+        return json.load(fh)[tag]
+
+
+def _solver(n, p, ppw, precision="fp64", bufc=1.0, box=(1.0, 1.0), center=(-0.1, 0.5)):
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    kh = O.kh_from_ppw(n, ppw)
+    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_constant(xx, kh))
+    a = 1 / (2 * (n / (p - 2)))
+    dom = Domain_Solver(torch.tensor([[0, box[0]], [0, box[1]]]), op, kh, a, p=p, buf_constant=bufc,
+                        precision=precision)
+    uu_dir = lambda xx: bif.uu_dir_func_greens(xx, kh, center=center)
+    return dom, kh, a, uu_dir
+
+
+def _oracle(n, p, ppw, bufc=1.0, box=(1.0, 1.0), center=(-0.1, 0.5)):
+    kh = O.kh_from_ppw(n, ppw)
+    a = 1 / (2 * (n / (p - 2)))
+    hps = O.HPSOracle(O.helmholtz_pdo(kh), np.array([[0, box[0]], [0, box[1]]]), a, p)
+    part = O.SlabPartition(hps.n0, hps.n1, p, bufc)
+    r = O.full_solve(hps, part, lambda xx: O.uu_dir_func_greens(xx, kh, center=center))
+    return hps, part, r
+
+
+@pytest.mark.parametrize("tag", ["small_superLU", "p16_superLU"])
+def test_fp64_vs_reference_solution(tag):
+    F = _fig(tag)
+    sols = np.load(os.path.join(GOLD, "reference_solutions.npz"))
+    dom, kh, a, uu_dir = _solver(F["n"], F["p"], F["ppw"])
+    print("case", tag, flush=True)
+    dom.build(verbose=False)
+    sol, res, true_err, resloc, toc = dom.solve(uu_dir, known_sol=True)
+    ref = sols[tag]
+    err = np.linalg.norm(sol.numpy().ravel() - ref) / np.linalg.norm(ref)
+    print(tag, "rel err vs reference solution", err, "true_err", true_err, F["trueres"], flush=True)
+    assert err < 1e-10
+    assert abs(true_err - F["trueres"]) < 1e-11
+    assert res < 1e-11
+
+
+def test_dtn_parity_test2d():
+    n, p, ppw = 400, 22, 40
+    dom, kh, a, uu_dir = _solver(n, p, ppw)
+    dom.hps.factor_leaves()
+    kh = O.kh_from_ppw(n, ppw)
+    hps = O.HPSOracle(O.helmholtz_pdo(kh), np.array([[0, 1.0], [0, 1.0]]), a, p)
+    boxes = np.array([0, 1, 57, 211, 399])
+    ref = hps.dtns(boxes)
+    got = dom.hps.leaf["DtN"][torch.as_tensor(boxes)].cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("DtN rel err", err, flush=True)
+    assert err < 1e-12
+    assert int(dom.hps.leaf["info"].abs().max()) == 0
+
+
+def test_fp64_test2d_square_end_to_end():
+    """tests/test_2d.py square case (n=400, p=22, ppw=40): solution vs the
+    oracle's exact discrete solution and known-solution error vs the
+    reference's figures."""
+    n, p, ppw = 400, 22, 40
+    dom, kh, a, uu_dir = _solver(n, p, ppw)
+    info = dom.build(verbose=True)
+    sol, res, true_err, resloc, toc = dom.solve(uu_dir, known_sol=True)
+    sol, res, true_err, resloc, toc = dom.solve(uu_dir, known_sol=True)
+    hps, part, r = _oracle(n, p, ppw)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("test2d: rel err vs oracle", err, "true_err", true_err, "res", res, "resloc", resloc,
+          "build", info["toc_build"], "solve", toc, flush=True)
+    assert err < 1e-10
+    assert true_err < 2e-5
+    assert abs(true_err - _fig("test2d_square_superLU")["trueres"]) < 1e-11
+    assert abs(true_err - r["true_err"]) < 1e-11
+    assert res < 1e-11
+    assert resloc < 10 * r["resloc"] + 1e-6
+    assert dom.test_slab_solve() < 1e-11
+
+
+def test_fp64_nonsquare_and_center():
+    """Rectangular domain, Dirichlet source at (1.5, 1.5)."""
+    n, p, ppw = 80, 10, 10
+    dom, kh, a, uu_dir = _solver(n, p, ppw, box=(1.5, 1.0), center=(1.5, 1.5))
+    dom.build(verbose=False)
+    sol, res, true_err, resloc, toc = dom.solve(uu_dir, known_sol=True)
+    hps, part, r = _oracle(n, p, ppw, box=(1.5, 1.0), center=(1.5, 1.5))
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("nonsquare rel err", err, true_err, r["true_err"], flush=True)
+    assert err < 1e-10
+    assert abs(true_err - r["true_err"]) < 1e-11
+
+
+def test_multi_rhs_matches_single():
+    from slablu_amd.src_disc import built_in_funcs as bif
+    n, p, ppw = 80, 10, 10
+    dom, kh, a, _ = _solver(n, p, ppw)
+    dom.build(verbose=False)
+    centers = [(1.3, 0.2), (-0.4, 1.1), (0.5, -0.6)]
+    multi = lambda xx: bif.uu_dir_func_greens_multi(xx, kh, centers)
+    solm, resm, tem, _, _ = dom.solve(multi, known_sol=True)
+    assert solm.shape[1] == 3
+    for k, c in enumerate(centers):
+        s1, _, _, _, _ = dom.solve(lambda xx: bif.uu_dir_func_greens(xx, kh, center=c))
+        assert torch.linalg.norm(solm[:, k] - s1[:, 0]) / torch.linalg.norm(s1) < 1e-12
+
+
+def test_mixed_refined_vs_fp64_oracle():
+    n, p, ppw = 400, 22, 40
+    dom, kh, a, uu_dir = _solver(n, p, ppw, precision="mixed")
+    dom.build(verbose=True)
+    sol, res, true_err, resloc, toc = dom.solve(uu_dir, known_sol=True)
+    hps, part, r = _oracle(n, p, ppw)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("mixed: rel err vs FP64 oracle", err, "refinement", dom.refine_history, "true_err", true_err, flush=True)
+    assert err < 1e-8
+    assert abs(true_err - r["true_err"]) < 1e-9
