@@ -93,6 +93,33 @@ int slab_btri_solve(int dtype, int b, int m, int bs,
                     const int* ipiv, int64_t p_bstride, int64_t p_istride,
                     void* f, int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream);
 
+/* Block-tridiagonal factorization in block-Thomas form with explicit
+ * diagonal inverses (same chains and storage as slab_btri_factor):
+ *   S_0 = D_0, S_i = D_i - sub_{i-1} G_{i-1}, G_i = S_i^{-1} sup_i;
+ * diag_i <- S_i^{-1}, sup_i <- G_i, sub_i unchanged.  This is the Schur
+ * recursion of src_solver/block_tridiag_solver_structured.py:34-76 with
+ * the inverse kept instead of the LU, so that every sweep step is a matrix
+ * product.  Blocks up to slab_btri_inv_fused(dtype, bs) == 1 run as one
+ * workgroup per chain (Gauss-Jordan inverse in LDS); otherwise ws must hold
+ * 2*b*bs*bs elements and iws b*bs ints. */
+int slab_btri_inv_fused(int dtype, int bs);
+int slab_btri_inv_factor(int dtype, int b, int m, int bs,
+                         void* diag, int64_t d_bstride, int64_t d_istride,
+                         void* sub, int64_t l_bstride, int64_t l_istride,
+                         void* sup, int64_t u_bstride, int64_t u_istride,
+                         int* info, void* ws, int* iws, void* stream);
+
+/* Sweeps with the factors of slab_btri_inv_factor, in place on f (as
+ * slab_btri_solve, non-transposed).  Replaces lusolve_sweep
+ * (src_solver/block_tridiag_solver_dense.py:179-202) and
+ * HBS_BlockTridiagonal solve (block_tridiag_solver_structured.py:157-187).
+ * For non-fused block sizes ws must hold b*bs*nrhs elements. */
+int slab_btri_inv_solve(int dtype, int b, int m, int bs,
+                        const void* diag, int64_t d_bstride, int64_t d_istride,
+                        const void* sub, int64_t l_bstride, int64_t l_istride,
+                        const void* sup, int64_t u_bstride, int64_t u_istride,
+                        void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* ws, void* stream);
+
 /* Leaf collocation rows for static condensation, all leaves in one launch:
  * M = [A_cc | A_cx] (border = 0) or [A_cc | A_cx ; N_c | N_x] (border = 1)
  * per leaf, from the scaled 1-D Chebyshev matrix D (p x p), Dsq = D@D and

@@ -178,6 +178,44 @@ int slab_btri_solve(int dtype, int b, int m, int bs, const void* diag, int64_t d
   return -1;
 }
 
+int slab_btri_inv_fused(int dtype, int bs) {
+  if (dtype == SLAB_F64) return btri_inv_fused_ok<double>(bs) ? 1 : 0;
+  if (dtype == SLAB_F32) return btri_inv_fused_ok<float>(bs) ? 1 : 0;
+  return -1;
+}
+
+int slab_btri_inv_factor(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride,
+                         void* sub, int64_t l_bstride, int64_t l_istride, void* sup, int64_t u_bstride,
+                         int64_t u_istride, int* info, void* ws, int* iws, void* stream) {
+  if (b < 0 || m < 1 || bs < 1) return -1;
+  if (dtype == SLAB_F64)
+    return status(btri_inv_factor<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub, l_bstride,
+                                          l_istride, (double*)sup, u_bstride, u_istride, info, (double*)ws, iws,
+                                          S(stream)));
+  if (dtype == SLAB_F32)
+    return status(btri_inv_factor<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
+                                         l_istride, (float*)sup, u_bstride, u_istride, info, (float*)ws, iws,
+                                         S(stream)));
+  return -1;
+}
+
+int slab_btri_inv_solve(int dtype, int b, int m, int bs, const void* diag, int64_t d_bstride, int64_t d_istride,
+                        const void* sub, int64_t l_bstride, int64_t l_istride, const void* sup, int64_t u_bstride,
+                        int64_t u_istride, void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* ws,
+                        void* stream) {
+  if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs) return -1;
+  if (nrhs == 0) return 0;
+  if (dtype == SLAB_F64)
+    return status(btri_inv_solve<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub, l_bstride,
+                                         l_istride, (double*)sup, u_bstride, u_istride, (double*)f, f_bstride, ldf,
+                                         nrhs, (double*)ws, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(btri_inv_solve<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
+                                        l_istride, (float*)sup, u_bstride, u_istride, (float*)f, f_bstride, ldf,
+                                        nrhs, (float*)ws, S(stream)));
+  return -1;
+}
+
 int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D, const double* Dsq,
                        const double* c11, const double* c22, const double* c12, const double* c1, const double* c2,
                        const double* c, const double* Nxtot, void* M, int64_t ldm, int64_t m_bs, void* stream) {

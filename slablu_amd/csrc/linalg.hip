@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int batch, int nrows, int k
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int b = blockIdx.x; b < batch; b += gridDim.x) {
     for (int e = tid; e < R * w; e += 256) {
-      const int r = e / w, c = e % w;
+      const int r = e / w, c = e - r * w;
       Ps[c * R + r] = *A.at(b, k0 + r, k0 + c);
     }
     __syncthreads();
@@ -308,24 +308,92 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int batch, int nrows, int k
       }
       __syncthreads();
       const T piv = Ps[j * R + j];
-      if (piv != T(0)) {
-        for (int r = j + 1 + tid; r < R; r += 256) Ps[j * R + r] = Ps[j * R + r] / piv;
-      } else if (tid == 0 && info != nullptr && info[b] == 0) {
-        info[b] = k0 + j + 1;
-      }
-      __syncthreads();
-      const int nr = R - j - 1, nc = w - j - 1;
-      if (nr > 0 && nc > 0) {
-        for (int e = tid; e < nr * nc; e += 256) {
-          const int c = j + 1 + e / nr, r = j + 1 + e % nr;
-          Ps[c * R + r] -= Ps[j * R + r] * Ps[c * R + j];
+      const bool nz = piv != T(0);
+      if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+      // each thread scales and updates whole rows: no index arithmetic in
+      // the inner loop, the pivot row is a broadcast read
+      for (int r = j + 1 + tid; r < R; r += 256) {
+        T l = Ps[j * R + r];
+        if (nz) {
+          l = l / piv;
+          Ps[j * R + r] = l;
         }
+        for (int c = j + 1; c < w; ++c) Ps[c * R + r] -= l * Ps[c * R + j];
       }
       __syncthreads();
     }
     for (int e = tid; e < R * w; e += 256) {
-      const int r = e / w, c = e % w;
+      const int r = e / w, c = e - r * w;
       *A.at(b, k0 + r, k0 + c) = Ps[c * R + r];
+    }
+    __syncthreads();
+  }
+}
+
+// Row interchanges of one LU panel (rows k0..k0+w-1 with 1-based pivots
+// ipiv[k0..k0+w-1] >= row, w <= 32) on every column outside [skip0, skip1).
+// The swap sequence is replayed once on row ids; each column then moves its
+// (at most 2w) touched entries with independent loads and stores.
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_panel(int batch, int ncols, int k0, int w, View<T> A,
+                                                     const int* ipiv, int64_t pbs, int skip0, int skip1) {
+  __shared__ int s_slot[32];
+  __shared__ int s_row[64];
+  __shared__ int s_src[64];
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  const int col = blockIdx.x * 256 + tid;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    // 1. slots: block rows k0+i -> i; pivot rows below the block -> w, w+1,
+    //    ... in order of first occurrence
+    bool first = false;
+    int p = -1, src_lane = 0;
+    unsigned long long mask = 0;
+    if (tid < 64) {
+      const int i = tid;
+      p = (i < w) ? ipiv[(int64_t)b * pbs + k0 + i] - 1 : -1;
+      first = (i < w) && (p >= k0 + w);
+      src_lane = i;
+      for (int q = 0; q < i && q < w; ++q) {
+        const int pq = ipiv[(int64_t)b * pbs + k0 + q] - 1;
+        if (first && pq == p) { first = false; src_lane = q; }
+      }
+      mask = __ballot(first);
+      const int rank = __popcll(mask & ((1ull << i) - 1ull));
+      if (first) {
+        s_slot[i] = w + rank;
+        s_row[w + rank] = p;
+      }
+      if (i < w) s_row[i] = k0 + i;
+      if (i == 0) s_cnt = w + __popcll(mask);
+    }
+    __syncthreads();
+    if (tid < w) {
+      if (p < k0 + w) s_slot[tid] = p - k0;
+      else if (!first) s_slot[tid] = s_slot[src_lane];
+    }
+    __syncthreads();
+    // 2. replay the swap sequence on slots
+    if (tid == 0) {
+      const int cnt = s_cnt;
+      for (int q = 0; q < cnt; ++q) s_src[q] = q;
+      for (int q = 0; q < w; ++q) {
+        const int c = s_slot[q];
+        const int t = s_src[q];
+        s_src[q] = s_src[c];
+        s_src[c] = t;
+      }
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (col < ncols && !(col >= skip0 && col < skip1)) {
+      T v[64];
+#pragma unroll
+      for (int q = 0; q < 64; ++q)
+        if (q < cnt) v[q] = *A.at(b, s_row[s_src[q]], col);
+#pragma unroll
+      for (int q = 0; q < 64; ++q)
+        if (q < cnt) *A.at(b, s_row[q], col) = v[q];
     }
     __syncthreads();
   }
@@ -359,7 +427,7 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
     // interchanges on every column outside the panel (the panel kernel
     // already swapped its own columns), then U12 = L11^{-1} A12
     dim3 g2(ceil_div(ncols, 256), grid_batch(batch));
-    hipLaunchKernelGGL(k_laswp<T>, g2, dim3(256), 0, s, batch, ncols, k0, k0 + w, 1, A, ipiv, pbs, k0, k0 + w);
+    hipLaunchKernelGGL(k_laswp_panel<T>, g2, dim3(256), 0, s, batch, ncols, k0, w, A, ipiv, pbs, k0, k0 + w);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int mr = nrows - k0 - w, nr = ncols - k0 - w;
     if (nr > 0) {
@@ -376,9 +444,49 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_copy(int batch, int rows, int cols, View<T> src, View<T> dst) {
+  const int64_t n = (int64_t)rows * cols;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y)
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+      const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+      *dst.at(b, r, c) = *src.at(b, r, c);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_identity(int batch, int n, View<T> dst) {
+  const int64_t nn = (int64_t)n * n;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y)
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+      const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
+      *dst.at(b, r, c) = (r == c) ? T(1) : T(0);
+    }
+}
+
+template <typename T>
+hipError_t copy_batched(int batch, int rows, int cols, View<T> src, View<T> dst, hipStream_t s) {
+  if (batch <= 0 || rows <= 0 || cols <= 0) return hipSuccess;
+  const int64_t n = (int64_t)rows * cols;
+  const int gx = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k_copy<T>, dim3(gx, grid_batch(batch)), dim3(256), 0, s, batch, rows, cols, src, dst);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  const int64_t nn = (int64_t)n * n;
+  const int gx = (int)((nn + 255) / 256 < 4096 ? (nn + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k_identity<T>, dim3(gx, grid_batch(batch)), dim3(256), 0, s, batch, n, dst);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
                  hipStream_t s) {
   hipError_t e;
+  if (!trans && nrhs <= 8 && n <= 2048 && sizeof(T) * ((size_t)n * nrhs + 64 * 65) + sizeof(int) * n <= 160 * 1024)
+    return getrs_small<T>(batch, n, nrhs, LU, ipiv, pbs, B, s);
   if (!trans) {
     if ((e = laswp<T>(batch, nrhs, 0, n, true, B, ipiv, pbs, s)) != hipSuccess) return e;
     if ((e = trsm_left<T>(batch, n, nrhs, true, true, LU, B, s)) != hipSuccess) return e;
@@ -394,7 +502,9 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
   template hipError_t trsm_left<T>(int, int, int, bool, bool, View<T>, View<T>, hipStream_t);                     \
   template hipError_t laswp<T>(int, int, int, int, bool, View<T>, const int*, int64_t, hipStream_t);              \
   template hipError_t getrf<T>(int, int, int, int, int, View<T>, int*, int64_t, int*, hipStream_t);               \
-  template hipError_t getrs<T>(int, int, int, bool, View<T>, const int*, int64_t, View<T>, hipStream_t);
+  template hipError_t getrs<T>(int, int, int, bool, View<T>, const int*, int64_t, View<T>, hipStream_t);         \
+  template hipError_t copy_batched<T>(int, int, int, View<T>, View<T>, hipStream_t);                              \
+  template hipError_t set_identity<T>(int, int, View<T>, hipStream_t);
 SLAB_INSTANTIATE(double)
 SLAB_INSTANTIATE(float)
 

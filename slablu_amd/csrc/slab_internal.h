@@ -60,3 +60,27 @@ hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const d
                          const double* c22, const double* c12, const double* c1, const double* c2, const double* c,
                          const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s);
 }  // namespace slab
+
+namespace slab {
+// dst(b) = src(b) for rows x cols views
+template <typename T>
+hipError_t copy_batched(int batch, int rows, int cols, View<T> src, View<T> dst, hipStream_t s);
+// dst(b) = I (n x n)
+template <typename T>
+hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
+// getrs for nrhs <= 8, non-transposed, one workgroup per matrix
+template <typename T>
+hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
+                       hipStream_t s);
+// block-Thomas factorization / sweeps with explicit diagonal inverses
+template <typename T>
+bool btri_inv_fused_ok(int bs);
+template <typename T>
+hipError_t btri_inv_factor(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                           int64_t lis, T* sup, int64_t ubs, int64_t uis, int* info, T* ws, int* iws,
+                           hipStream_t s);
+template <typename T>
+hipError_t btri_inv_solve(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                          int64_t lis, T* sup, int64_t ubs, int64_t uis, T* f, int64_t fbs, int64_t ldf, int nrhs,
+                          T* ws, hipStream_t s);
+}  // namespace slab

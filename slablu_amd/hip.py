@@ -39,6 +39,11 @@ def _load():
                               _i64, _vp, _vp], _int),
         "slab_btri_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                              _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
+        "slab_btri_inv_fused": ([_int, _int], _int),
+        "slab_btri_inv_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                                  _vp, _vp, _vp], _int),
+        "slab_btri_inv_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                                 _i64, _i64, _int, _vp, _vp], _int),
         "slab_leaf_assemble": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                 _vp], _int),
     }
@@ -184,6 +189,48 @@ def btri_solve(diag, sub, sup, ipiv, f, transpose=False):
                                sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
                                sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(f),
                                f.stride(0), f.stride(1), f.shape[2], int(transpose), _stream()), "slab_btri_solve")
+    return f
+
+
+def btri_inv_fused(dtype, bs):
+    return LIB.slab_btri_inv_fused(F64 if dtype == torch.float64 else F32, bs) == 1
+
+
+def _blk_strides(t, m):
+    return t.stride(0), (t.stride(1) if m > 1 else 0)
+
+
+def btri_inv_factor(diag, sub, sup, info=None):
+    """Block-Thomas factorization in place (diag_i <- S_i^{-1}, sup_i <- G_i);
+    diag (b, m, bs, bs), sub/sup (b, >= m-1, bs, bs), contiguous blocks."""
+    b, m, bs, _ = diag.shape
+    for t in (diag, sub, sup):
+        _require(t.stride(-1) == 1 and t.stride(-2) == bs and t.dtype == diag.dtype, "btri block layout")
+    ws = iws = None
+    if not btri_inv_fused(diag.dtype, bs):
+        ws = torch.empty(2 * b * bs * bs, dtype=diag.dtype, device=diag.device)
+        iws = torch.empty(b * bs, dtype=torch.int32, device=diag.device)
+    ds, di = diag.stride(0), diag.stride(1)
+    ls, li = _blk_strides(sub, m)
+    us, ui = _blk_strides(sup, m)
+    _check(LIB.slab_btri_inv_factor(_dt(diag), b, m, bs, _p(diag), ds, di, _p(sub), ls, li, _p(sup), us, ui, _p(info),
+                                    _p(ws), _p(iws), _stream()), "slab_btri_inv_factor")
+
+
+def btri_inv_solve(diag, sub, sup, f):
+    """In-place sweeps on f (b, m*bs, nrhs) with the factors of btri_inv_factor."""
+    b, m, bs, _ = diag.shape
+    _require(f.dim() == 3 and f.shape[0] == b and f.shape[1] == m * bs and f.stride(2) == 1, "btri rhs layout")
+    _require(f.dtype == diag.dtype, "btri rhs dtype")
+    nrhs = f.shape[2]
+    ws = None
+    if not btri_inv_fused(diag.dtype, bs):
+        ws = torch.empty(b * bs * max(nrhs, 1), dtype=diag.dtype, device=diag.device)
+    ds, di = diag.stride(0), diag.stride(1)
+    ls, li = _blk_strides(sub, m)
+    us, ui = _blk_strides(sup, m)
+    _check(LIB.slab_btri_inv_solve(_dt(diag), b, m, bs, _p(diag), ds, di, _p(sub), ls, li, _p(sup), us, ui, _p(f),
+                                   f.stride(0), f.stride(1), nrhs, _p(ws), _stream()), "slab_btri_inv_solve")
     return f
 
 

@@ -2,14 +2,18 @@
 src_solver/block_tridiag_solver_dense.py of the reference).
 
 Storage layout is the reference's Tdata: (b, m + 2(m-1) + m//bs + 1, bs, bs)
-= [diag (m) | sub (m-1) | sup (m-1) | pivot blocks], after factorization
-diag_i holds the packed LU of D_i - Lfac_i Ufac_i, sub_i holds
-Lfac = sub_i U_i^{-1} and sup_i holds Ufac = L_i^{-1} P_i^T sup_i
-(block_tridiag_solver_dense.py:152-177).  Pivots are additionally kept as an
-int32 tensor for the kernels and packed into the pivot blocks as values
-(1-based, as LAPACK) for layout compatibility.  The factorization and the
-sweeps run in libslabhip (slab_btri_factor / slab_btri_solve); Tdata stays
-on the device.
+= [diag (m) | sub (m-1) | sup (m-1) | pivot blocks] (block_tridiag_solver_dense.py
+:226-257, 291-330), kept on the device.
+
+form='inv' (default): block-Thomas factorization with explicit diagonal
+inverses, diag_i <- S_i^{-1} with S_i = D_i - sub_{i-1} G_{i-1}, sup_i <- G_i =
+S_i^{-1} sup_i, sub_i unchanged (slab_btri_inv_factor / slab_btri_inv_solve):
+every sweep step is a matrix product, and blocks that fit in LDS run as one
+workgroup per chain.
+form='lu': the reference's block LU, diag_i <- packed LU of D_i - Lfac_i
+Ufac_i, sub_i <- Lfac = sub_i U_i^{-1}, sup_i <- Ufac = L_i^{-1} P_i^T sup_i
+(block_tridiag_solver_dense.py:152-177), pivots packed into the pivot blocks
+as values (1-based); needed for transposed solves.
 """
 import numpy as np
 import torch
@@ -38,8 +42,11 @@ def unpack_pivs(b, m, bs, Tdata, device=None):
 
 
 class Block_Tridiagonal:
-    def __init__(self, b, m, bs):
+    def __init__(self, b, m, bs, form="inv"):
+        if form not in ("inv", "lu"):
+            raise ValueError("form must be 'inv' or 'lu'")
         self.b, self.m, self.bs = b, m, bs
+        self.form = form
         self.pivdata_nblocks = int(m / bs) + 1
         self.storage_blocks = m + 2 * (m - 1) + self.pivdata_nblocks
         self.nbytes = (self.b * self.storage_blocks * self.bs ** 2) * 8
@@ -70,9 +77,12 @@ class Block_Tridiagonal:
         self.itemsize = self.Tdata.element_size()
         self.nbytes = self.Tdata.numel() * self.itemsize
         b, m, bs = self.b, self.m, self.bs
-        self.pivs = torch.zeros(b, m, bs, dtype=torch.int32, device=self.Tdata.device)
         self.info = torch.zeros(b, dtype=torch.int32, device=self.Tdata.device)
         d, l, u = self._views()
+        if self.form == "inv":
+            hip.btri_inv_factor(d, l, u, info=self.info)
+            return
+        self.pivs = torch.zeros(b, m, bs, dtype=torch.int32, device=self.Tdata.device)
         hip.btri_factor(d, l, u, self.pivs, info=self.info)
         pack_pivs(b, m, bs, self.Tdata, self.pivs)
 
@@ -86,7 +96,12 @@ class Block_Tridiagonal:
         fv = f.reshape(b_end - b_start, self.m * self.bs, -1)
         if fv.stride(-1) != 1:
             raise ValueError("rhs must have unit column stride")
-        hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose)
+        if self.form == "inv":
+            if transpose:
+                raise NotImplementedError("transposed sweeps need form='lu'")
+            hip.btri_inv_solve(d, l, u, fv)
+        else:
+            hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose)
         return fv
 
 

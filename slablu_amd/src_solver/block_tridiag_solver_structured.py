@@ -3,10 +3,11 @@ src_solver/block_tridiag_solver_structured.py of the reference).
 
 The reference keeps the interface blocks HBS-compressed and densifies each
 one before its LU (block_tridiag_solver_structured.py:34-76).  Here the
-blocks are formed exactly and densely on the GPU, so the Schur recursion
-S_0 = D_0, S_j = D_j - sub_{j-1} S_{j-1}^{-1} sup_{j-1} with LU(S_j) is the
-block LU of a single (b = 1) block-tridiagonal chain, run by
-slab_btri_factor in FP64.
+blocks are formed exactly and densely on the GPU, and the Schur recursion
+S_0 = D_0, S_j = D_j - sub_{j-1} S_{j-1}^{-1} sup_{j-1} is run as one
+(b = 1) block-tridiagonal chain in block-Thomas form with S_j^{-1} and
+G_j = S_j^{-1} sup_j kept (slab_btri_inv_factor, FP64), so that the solve
+is a sequence of matrix products.
 """
 import torch
 
@@ -43,19 +44,18 @@ class HBS_BlockTridiagonal:
         else:
             self.sub = torch.zeros(1, 1, n, n, device=device, dtype=dtype)
             self.sup = torch.zeros(1, 1, n, n, device=device, dtype=dtype)
-        self.pivs = torch.zeros(1, b, n, dtype=torch.int32, device=device)
         self.info = torch.zeros(1, dtype=torch.int32, device=device)
-        hip.btri_factor(self.diag, self.sub, self.sup, self.pivs, info=self.info)
+        hip.btri_inv_factor(self.diag, self.sub, self.sup, info=self.info)
         self.nbytes = (self.diag.numel() + self.sub.numel() + self.sup.numel()) * self.diag.element_size()
         self.build_mode = "lowmem"
 
     build = build_lowmem
 
-    def solve(self, f, transpose=False):
+    def solve(self, f):
         """f (b*n, nrhs) -> solution (b*n, nrhs); f is not modified."""
         nrhs = f.shape[-1]
         x = f.to(self.diag.device, self.diag.dtype).reshape(1, self.b * self.n, nrhs).contiguous().clone()
-        hip.btri_solve(self.diag, self.sub, self.sup, self.pivs, x, transpose=transpose)
+        hip.btri_inv_solve(self.diag, self.sub, self.sup, x)
         return x.reshape(self.b * self.n, nrhs)
 
     solve_lowmem = solve

@@ -232,3 +232,90 @@ def test_leaf_assemble_bit_exact(p, general):
     ref = A[:, Jc][:, :, Jtot]
     assert np.array_equal(Mc[:, :ni], ref)
     assert np.array_equal(Mc[:, ni:], np.broadcast_to(hps.Nx[:, Jtot], (nb, nx, ni + nx)))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("b,m,bs", [(5, 4, 20), (3, 7, 100), (2, 3, 154), (1, 3, 200), (4, 1, 33), (2, 2, 16)])
+@pytest.mark.parametrize("nrhs", [1, 7, 40])
+def test_btri_inv_factor_solve(dtype, b, m, bs, nrhs):
+    """Block-Thomas factorization with explicit inverses (fused in LDS for
+    small blocks, batched getrf/getrs/gemm otherwise) vs a dense solve."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(b * 1000 + m * 10 + bs + nrhs)
+    # diagonal shift ~ 2 sqrt(bs) keeps the blocks well conditioned at every size
+    shift = 3 + 2 * bs ** 0.5
+    diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + shift * torch.eye(bs, dtype=torch.float64)
+    sub0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    sup0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
+    diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
+    info = torch.zeros(b, dtype=torch.int32, device="cuda")
+    hip.btri_inv_factor(diag, sub, sup, info=info)
+    f = f0.to("cuda", dtype).contiguous()
+    hip.btri_inv_solve(diag, sub, sup, f)
+    M = _btri_dense(diag0, sub0[:, :m - 1], sup0[:, :m - 1])
+    ref = torch.linalg.solve(M, f0)
+    torch.cuda.synchronize()
+    assert int(info.abs().max()) == 0
+    assert _rel(f, ref) < 200 * TOL[dtype]
+    # the diagonal slot holds S_0^{-1} = D_0^{-1}
+    assert _rel(diag[:, 0], torch.linalg.inv(diag0[:, 0])) < 200 * TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [33, 100, 400])
+@pytest.mark.parametrize("nrhs", [1, 3, 8])
+def test_getrs_small_rhs(dtype, n, nrhs):
+    hip = _hip()
+    g = torch.Generator().manual_seed(n + nrhs)
+    b = 5
+    A0 = torch.randn(b, n, n, generator=g, dtype=torch.float64) + 2 * torch.eye(n, dtype=torch.float64)
+    B0 = torch.randn(b, n, nrhs, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype).contiguous()
+    ipiv = torch.zeros(b, n, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv)
+    X = B0.to("cuda", dtype).contiguous()
+    hip.getrs(A, ipiv, X)
+    torch.cuda.synchronize()
+    assert _rel(X, torch.linalg.solve(A0, B0)) < 500 * TOL[dtype]
+
+
+@pytest.mark.parametrize("form", ["inv", "lu"])
+def test_block_tridiagonal_api(form):
+    """Block_Tridiagonal(b, m, bs) drop-in: build_sweep / lusolve_sweep on
+    slices of the batch, as slablu_solver.py uses it."""
+    from slablu_amd.src_solver.block_tridiag_solver_dense import Block_Tridiagonal
+    g = torch.Generator().manual_seed(7)
+    b, m, bs, nrhs = 6, 5, 20, 3
+    diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 4 * torch.eye(bs, dtype=torch.float64)
+    sub0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+    sup0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+    T = Block_Tridiagonal(b, m, bs, form=form)
+    T.build_sweep(diag0.cuda(), sub0.cuda(), sup0.cuda())
+    assert T.Tdata.shape == (b, T.storage_blocks, bs, bs)
+    f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
+    M = _btri_dense(diag0, sub0, sup0)
+    x = T.lusolve_sweep(f0[2:5].cuda().clone(), b_start=2, b_end=5)
+    torch.cuda.synchronize()
+    assert _rel(x, torch.linalg.solve(M[2:5], f0[2:5])) < 1e-11
+    if form == "lu":
+        xt = T.lusolve_sweep(f0.cuda().clone(), transpose=True)
+        torch.cuda.synchronize()
+        assert _rel(xt, torch.linalg.solve(M.transpose(1, 2), f0)) < 1e-11
+
+
+def test_hbs_block_tridiagonal_api():
+    from slablu_amd.src_solver.block_tridiag_solver_structured import HBS_BlockTridiagonal
+    g = torch.Generator().manual_seed(9)
+    nb, n = 4, 300
+    d0 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) + 5 * torch.eye(n, dtype=torch.float64)
+    d1 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) * 0.1
+    sub = torch.randn(nb - 1, n, n, generator=g, dtype=torch.float64) * 0.2
+    sup = torch.randn(nb - 1, n, n, generator=g, dtype=torch.float64) * 0.2
+    H = HBS_BlockTridiagonal(nb, n)
+    H.build_lowmem(d0, d1, sub, sup, device="cuda")
+    f = torch.randn(nb * n, 2, generator=g, dtype=torch.float64)
+    x = H.solve(f.cuda())
+    M = _btri_dense((d0 + d1)[None], sub[None], sup[None])[0]
+    torch.cuda.synchronize()
+    assert _rel(x, torch.linalg.solve(M, f)) < 1e-11
