@@ -19,6 +19,9 @@ template <typename T>
 hipError_t btri_factor(int b, int m, int bs, void* diag, int64_t dbs, int64_t dis, void* sub, int64_t lbs,
                        int64_t lis, void* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis,
                        int* info, hipStream_t s) {
+  if (btri_lu_fused_ok<T>(bs))
+    return btri_lu_factor_fused<T>(b, m, bs, (T*)diag, dbs, dis, (T*)sub, lbs, lis, (T*)sup, ubs, uis, ipiv, pbs,
+                                   pis, info, s);
   hipError_t e;
   e = getrf<T>(b, bs, bs, bs, bs, blk<T>(diag, dbs, dis, 0, bs), ipiv, pbs, info, s);
   if (e != hipSuccess) return e;
@@ -50,6 +53,9 @@ hipError_t btri_solve(int b, int m, int bs, const void* diag, int64_t dbs, int64
   auto F = [&](int i) { return View<T>{reinterpret_cast<T*>(f) + (int64_t)i * bs * ldf, ldf, 1, fbs}; };
   auto P = [&](int i) { return ipiv + (int64_t)i * pis; };
   hipError_t e;
+  if (!trans && btri_lu_fused_ok<T>(bs))
+    return btri_lu_solve_fused<T>(b, m, bs, (T*)diag, dbs, dis, (T*)sub, lbs, lis, (T*)sup, ubs, uis, ipiv, pbs,
+                                  pis, (T*)f, fbs, ldf, nrhs, s);
   if (!trans) {
     for (int i = 0; i < m; ++i) {
       if (i > 0 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), L(i - 1), F(i - 1), T(1), F(i), s)) != hipSuccess) return e;

@@ -462,6 +462,330 @@ __global__ __launch_bounds__(256) void k_getrs_small(int batch, int n, int nrhs,
   }
 }
 
+// ---------------------------------------------------------------------------
+// LU form (backward stable; the reference's block LU): diag_i <- packed LU of
+// D_i - Lfac_i Ufac_i, sub_i <- Lfac = sub_i U_i^{-1}, sup_i <- Ufac =
+// L_i^{-1} P_i^T sup_i, ipiv (b, m, bs) 1-based.
+// ---------------------------------------------------------------------------
+
+// Lower triangular solve M X = X in LDS, X n x nc (row stride nc), M(r, k)
+// read through `at` (LDS or global, see the callers); 64-row blocks: each
+// wavefront substitutes its columns with shuffles (lane = row), then all
+// threads update the rows below.  `upper` runs the mirrored algorithm from
+// the bottom.  Dg (64 x 65, LDS) stages the diagonal block when M is global.
+template <typename T, typename At>
+__device__ void tri_solve_lds(At at, int n, bool upper, bool unit, T* X, int nc, T* Dg) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + 63) / 64;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = upper ? nblk - 1 - bi : bi;
+    const int d0 = blk * 64;
+    const int tb = (n - d0) < 64 ? (n - d0) : 64;
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      Dg[r * 65 + c] = (r < tb && c < tb) ? at(d0 + r, d0 + c) : T(0);
+    }
+    __syncthreads();
+    for (int c = wave; c < nc; c += 4) {
+      T x = (lane < tb) ? X[(d0 + lane) * nc + c] : T(0);
+      if (!upper) {
+        for (int j = 0; j < tb; ++j) {
+          if (!unit && lane == j) x = x / Dg[j * 65 + j];
+          const T xj = __shfl(x, j);
+          if (lane > j) x -= Dg[lane * 65 + j] * xj;
+        }
+      } else {
+        for (int j = tb - 1; j >= 0; --j) {
+          if (!unit && lane == j) x = x / Dg[j * 65 + j];
+          const T xj = __shfl(x, j);
+          if (lane < j) x -= Dg[lane * 65 + j] * xj;
+        }
+      }
+      if (lane < tb) X[(d0 + lane) * nc + c] = x;
+    }
+    __syncthreads();
+    const int r0 = upper ? 0 : d0 + tb;
+    const int r1 = upper ? d0 : n;
+    const int nrow = r1 - r0;
+    for (int e = tid; e < nrow * nc; e += 256) {
+      const int rr = e / nc, c = e - rr * nc;
+      const int r = r0 + rr;
+      T acc = T(0);
+      for (int j = 0; j < tb; ++j) acc += at(r, d0 + j) * X[(d0 + j) * nc + c];
+      X[r * nc + c] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Row interchanges (forward) of a LU with pivots P (0-based, LDS) on X.
+template <typename T>
+__device__ void apply_piv_lds(const int* P, int n, T* X, int nc) {
+  for (int c = threadIdx.x; c < nc; c += 256)
+    for (int k = 0; k < n; ++k) {
+      const int p = P[k];
+      if (p != k) {
+        const T t = X[k * nc + c];
+        X[k * nc + c] = X[p * nc + c];
+        X[p * nc + c] = t;
+      }
+    }
+  __syncthreads();
+}
+
+// In-place LU with partial pivoting of S (n x n, LDS, row stride n+1).
+template <typename T, int RT>
+__device__ void lu_lds(T* S, int n, int* piv, T* red_v, int* red_i, int* s_piv, int* info, int64_t b) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int ld = n + 1;
+  for (int k = 0; k < n; ++k) {
+    T best = T(-1);
+    int bi = INT_MAX;
+    for (int r = k + tid; r < n; r += 256) {
+      const T v = absv(S[r * ld + k]);
+      if (v > best) { best = v; bi = r; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T bv = red_v[0];
+      int bx = red_i[0];
+      for (int q = 1; q < 4; ++q)
+        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) { bv = red_v[q]; bx = red_i[q]; }
+      if (bx < k || bx >= n) bx = k;
+      *s_piv = bx;
+      piv[k] = bx;
+    }
+    __syncthreads();
+    const int p = *s_piv;
+    if (p != k) {
+      for (int c = tid; c < n; c += 256) {
+        const T t = S[k * ld + c];
+        S[k * ld + c] = S[p * ld + c];
+        S[p * ld + c] = t;
+      }
+    }
+    __syncthreads();
+    const T pv = S[k * ld + k];
+    const bool nz = pv != T(0);
+    if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k + 1;
+    T rowk[RT], colk[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      const int c = tx + 16 * q;
+      rowk[q] = (c < n && c > k) ? S[k * ld + c] : T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+      const int r = ty + 16 * a;
+      colk[a] = (r < n && r > k) ? (nz ? S[r * ld + k] / pv : S[r * ld + k]) : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+      const int r = ty + 16 * a;
+      if (r < n && r > k) {
+        if (tx == (k & 15) && (k >> 4) < RT) S[r * ld + k] = colk[a];
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+          const int c = tx + 16 * q;
+          if (c < n && c > k) S[r * ld + c] -= colk[a] * rowk[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int RT>
+__global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, BtriArgs<T> a, int* ipiv,
+                                                        int64_t pbs, int64_t pis, int* info) {
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw);   // bs x (bs+1)
+  T* X = S + bs * (bs + 1);                 // bs x 32 (right-hand-side chunk)
+  T* TA = X + bs * 32;                      // bs x (KC+1)
+  T* TB = TA + bs * (KC + 1);               // KC x (bs+1); also the 64 x 65 diagonal stage
+  int* piv = reinterpret_cast<int*>(TB + (KC * (bs + 1) > 64 * 65 ? KC * (bs + 1) : 64 * 65));
+  __shared__ T red_v[4];
+  __shared__ int red_i[4];
+  __shared__ int s_piv;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int ld = bs + 1;
+  for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    for (int i = 0; i < m; ++i) {
+      T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
+      for (int e = tid; e < bs * bs; e += 256) {
+        const int r = e / bs, c = e - r * bs;
+        S[r * ld + c] = Di[e];
+      }
+      __syncthreads();
+      if (i > 0) {
+        T acc[RT][RT];
+#pragma unroll
+        for (int x = 0; x < RT; ++x)
+#pragma unroll
+          for (int y = 0; y < RT; ++y) acc[x][y] = T(0);
+        const T* Lp = a.sub + b * a.lbs + (int64_t)(i - 1) * a.lis;
+        const T* Up = a.sup + b * a.ubs + (int64_t)(i - 1) * a.uis;
+        mm_global_global<T, RT>(Lp, Up, bs, TA, TB, acc);
+#pragma unroll
+        for (int x = 0; x < RT; ++x) {
+          const int r = ty + 16 * x;
+#pragma unroll
+          for (int y = 0; y < RT; ++y) {
+            const int c = tx + 16 * y;
+            if (r < bs && c < bs) S[r * ld + c] -= acc[x][y];
+          }
+        }
+        __syncthreads();
+      }
+      lu_lds<T, RT>(S, bs, piv, red_v, red_i, &s_piv, info, b);
+      for (int e = tid; e < bs * bs; e += 256) {
+        const int r = e / bs, c = e - r * bs;
+        Di[e] = S[r * ld + c];
+      }
+      int* Pi = ipiv + b * pbs + (int64_t)i * pis;
+      for (int r = tid; r < bs; r += 256) Pi[r] = piv[r] + 1;
+      if (i < m - 1) {
+        T* Ui = a.sup + b * a.ubs + (int64_t)i * a.uis;
+        T* Li = a.sub + b * a.lbs + (int64_t)i * a.lis;
+        auto Lat = [&](int r, int k) -> T { return S[r * ld + k]; };
+        auto UTat = [&](int r, int k) -> T { return S[k * ld + r]; };
+        for (int c0 = 0; c0 < bs; c0 += 32) {
+          const int nc = (bs - c0) < 32 ? (bs - c0) : 32;
+          // Ufac chunk: L^{-1} P sup_i[:, c0:c0+nc]
+          for (int e = tid; e < bs * nc; e += 256) {
+            const int r = e / nc, c = e - r * nc;
+            X[e] = Ui[(int64_t)r * bs + c0 + c];
+          }
+          __syncthreads();
+          apply_piv_lds<T>(piv, bs, X, nc);
+          tri_solve_lds<T>(Lat, bs, false, true, X, nc, TB);
+          for (int e = tid; e < bs * nc; e += 256) {
+            const int r = e / nc, c = e - r * nc;
+            Ui[(int64_t)r * bs + c0 + c] = X[e];
+          }
+          __syncthreads();
+          // Lfac chunk: rows c0..c0+nc of sub_i U^{-1}, i.e. U^T Z = sub_i^T
+          for (int e = tid; e < bs * nc; e += 256) {
+            const int r = e / nc, c = e - r * nc;
+            X[e] = Li[(int64_t)(c0 + c) * bs + r];
+          }
+          __syncthreads();
+          tri_solve_lds<T>(UTat, bs, false, false, X, nc, TB);
+          for (int e = tid; e < bs * nc; e += 256) {
+            const int r = e / nc, c = e - r * nc;
+            Li[(int64_t)(c0 + c) * bs + r] = X[e];
+          }
+          __syncthreads();
+        }
+      }
+      __threadfence();
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv,
+                                                       int64_t pbs, int64_t pis, T* f, int64_t fbs, int64_t ldf,
+                                                       int nrhs, int NR) {
+  extern __shared__ unsigned char smem_raw[];
+  T* Xc = reinterpret_cast<T*>(smem_raw);  // bs x NR current block
+  T* Y = Xc + bs * NR;                     // bs x NR previous block
+  T* TA = Y + bs * NR;                     // bs x (KC+1)
+  T* Dg = TA + bs * (KC + 1);              // 64 x 65
+  int* P = reinterpret_cast<int*>(Dg + 64 * 65);
+  const int tid = threadIdx.x;
+  const int nr = NR;
+  const int tc = tid % nr, tr = tid / nr, RG = 256 / nr;
+  const int c0 = blockIdx.y * NR;
+  const int ncols = (nrhs - c0) < NR ? (nrhs - c0) : NR;
+  for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    T* fb = f + b * fbs;
+    auto F = [&](int i, int r, int c) -> T* { return fb + (int64_t)(i * bs + r) * ldf + c0 + c; };
+    for (int i = 0; i < m; ++i) {
+      for (int e = tid; e < bs * nr; e += 256) {
+        const int r = e / nr, c = e - r * nr;
+        Xc[e] = (c < ncols) ? *F(i, r, c) : T(0);
+      }
+      for (int r = tid; r < bs; r += 256) P[r] = ipiv[b * pbs + (int64_t)i * pis + r] - 1;
+      __syncthreads();
+      if (i > 0) {
+        T acc[RPT];
+#pragma unroll
+        for (int x = 0; x < RPT; ++x) acc[x] = T(0);
+        mv_global_lds<T, RPT>(a.sub + b * a.lbs + (int64_t)(i - 1) * a.lis, Y, bs, nr, TA, acc);
+#pragma unroll
+        for (int x = 0; x < RPT; ++x) {
+          const int r = tr + RG * x;
+          if (r < bs) Xc[r * nr + tc] -= acc[x];
+        }
+        __syncthreads();
+      }
+      apply_piv_lds<T>(P, bs, Xc, nr);
+      const T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
+      auto Lat = [&](int r, int k) -> T { return Di[(int64_t)r * bs + k]; };
+      tri_solve_lds<T>(Lat, bs, false, true, Xc, nr, Dg);
+      for (int e = tid; e < bs * nr; e += 256) {
+        const int r = e / nr, c = e - r * nr;
+        Y[e] = Xc[e];
+        if (c < ncols) *F(i, r, c) = Xc[e];
+      }
+      __syncthreads();
+    }
+    for (int i = m - 1; i >= 0; --i) {
+      if (i < m - 1) {
+        for (int e = tid; e < bs * nr; e += 256) {
+          const int r = e / nr, c = e - r * nr;
+          Xc[e] = (c < ncols) ? *F(i, r, c) : T(0);
+        }
+        __syncthreads();
+        T acc[RPT];
+#pragma unroll
+        for (int x = 0; x < RPT; ++x) acc[x] = T(0);
+        mv_global_lds<T, RPT>(a.sup + b * a.ubs + (int64_t)i * a.uis, Y, bs, nr, TA, acc);
+#pragma unroll
+        for (int x = 0; x < RPT; ++x) {
+          const int r = tr + RG * x;
+          if (r < bs) Xc[r * nr + tc] -= acc[x];
+        }
+        __syncthreads();
+      } else {
+        for (int e = tid; e < bs * nr; e += 256) Xc[e] = Y[e];
+        __syncthreads();
+      }
+      const T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
+      auto Uat = [&](int r, int k) -> T { return Di[(int64_t)r * bs + k]; };
+      tri_solve_lds<T>(Uat, bs, true, false, Xc, nr, Dg);
+      for (int e = tid; e < bs * nr; e += 256) {
+        const int r = e / nr, c = e - r * nr;
+        Y[e] = Xc[e];
+        if (c < ncols) *F(i, r, c) = Xc[e];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T>
+size_t lu_factor_lds(int bs) {
+  const size_t tb = (size_t)KC * (bs + 1) > 64 * 65 ? (size_t)KC * (bs + 1) : 64 * 65;
+  return sizeof(T) * ((size_t)bs * (bs + 1) + (size_t)bs * 32 + (size_t)bs * (KC + 1) + tb) + sizeof(int) * bs;
+}
+
+template <typename T>
+size_t lu_solve_lds(int bs, int nr) {
+  return sizeof(T) * (2 * (size_t)bs * nr + (size_t)bs * (KC + 1) + 64 * 65) + sizeof(int) * bs;
+}
+
 template <typename T>
 size_t factor_lds(int bs) {
   return sizeof(T) * ((size_t)bs * (bs + 1) + (size_t)bs * (KC + 1) + (size_t)KC * (bs + 1)) + sizeof(int) * bs;
@@ -494,6 +818,31 @@ hipError_t launch_solve(int nb, int m, int bs, BtriArgs<T> a, T* f, int64_t fbs,
   if (e != hipSuccess) return e;
   dim3 grid(nb < 65535 ? nb : 65535, (nrhs + nr - 1) / nr);
   hipLaunchKernelGGL((k_btri_inv_solve<T, RPT>), grid, dim3(256), lds, s, nb, m, bs, a, f, fbs, ldf, nrhs, nr);
+  return hipGetLastError();
+}
+
+template <typename T, int RT>
+hipError_t launch_lu_factor(int nb, int m, int bs, BtriArgs<T> a, int* ipiv, int64_t pbs, int64_t pis, int* info,
+                            hipStream_t s) {
+  const size_t lds = lu_factor_lds<T>(bs);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_factor<T, RT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_btri_lu_factor<T, RT>), dim3(nb < 65535 ? nb : 65535), dim3(256), lds, s, nb, m, bs, a,
+                     ipiv, pbs, pis, info);
+  return hipGetLastError();
+}
+
+template <typename T, int RPT>
+hipError_t launch_lu_solve(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv, int64_t pbs, int64_t pis, T* f,
+                           int64_t fbs, int64_t ldf, int nrhs, int nr, hipStream_t s) {
+  const size_t lds = lu_solve_lds<T>(bs, nr);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_solve<T, RPT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(nb < 65535 ? nb : 65535, (nrhs + nr - 1) / nr);
+  hipLaunchKernelGGL((k_btri_lu_solve<T, RPT>), grid, dim3(256), lds, s, nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf,
+                     nrhs, nr);
   return hipGetLastError();
 }
 
@@ -584,6 +933,44 @@ hipError_t btri_inv_solve(int nb, int m, int bs, T* diag, int64_t dbs, int64_t d
 }
 
 template <typename T>
+bool btri_lu_fused_ok(int bs) {
+  return (bs + 15) / 16 <= 10 && lu_factor_lds<T>(bs) <= kLdsMax;
+}
+
+template <typename T>
+hipError_t btri_lu_factor_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                                int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis,
+                                int* info, hipStream_t s) {
+  BtriArgs<T> a{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis};
+  const int rt = (bs + 15) / 16;
+  switch (rt) {
+    case 1: return launch_lu_factor<T, 1>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+    case 2: return launch_lu_factor<T, 2>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+    case 3: case 4: return launch_lu_factor<T, 4>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+    case 5: case 6: return launch_lu_factor<T, 6>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+    case 7: case 8: return launch_lu_factor<T, 8>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+    default: return launch_lu_factor<T, 10>(nb, m, bs, a, ipiv, pbs, pis, info, s);
+  }
+}
+
+template <typename T>
+hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                               int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
+                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s) {
+  BtriArgs<T> a{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis};
+  int nr = nrhs >= 32 ? 32 : (nrhs >= 16 ? 16 : (nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1))));
+  while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
+  const int rpt = (bs + 256 / nr - 1) / (256 / nr);
+  if (lu_solve_lds<T>(bs, nr) > kLdsMax || rpt > 24) return hipErrorInvalidValue;
+  if (rpt <= 1) return launch_lu_solve<T, 1>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  if (rpt <= 2) return launch_lu_solve<T, 2>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  if (rpt <= 4) return launch_lu_solve<T, 4>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  if (rpt <= 8) return launch_lu_solve<T, 8>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  if (rpt <= 13) return launch_lu_solve<T, 13>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  return launch_lu_solve<T, 24>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+}
+
+template <typename T>
 hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
                        hipStream_t s) {
   if (nrhs > 8) return hipErrorInvalidValue;
@@ -603,7 +990,13 @@ hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, 
                                          int64_t, int*, T*, int*, hipStream_t);                                    \
   template hipError_t btri_inv_solve<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*, int64_t,   \
                                         int64_t, T*, int64_t, int64_t, int, T*, hipStream_t);                      \
-  template hipError_t getrs_small<T>(int, int, int, View<T>, const int*, int64_t, View<T>, hipStream_t);
+  template hipError_t getrs_small<T>(int, int, int, View<T>, const int*, int64_t, View<T>, hipStream_t);          \
+  template bool btri_lu_fused_ok<T>(int);                                                                          \
+  template hipError_t btri_lu_factor_fused<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,      \
+                                              int64_t, int64_t, int*, int64_t, int64_t, int*, hipStream_t);        \
+  template hipError_t btri_lu_solve_fused<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,       \
+                                             int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t, int64_t, \
+                                             int, hipStream_t);
 SLAB_BTRI_INST(double)
 SLAB_BTRI_INST(float)
 

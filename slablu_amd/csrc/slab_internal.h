@@ -72,6 +72,17 @@ hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
 template <typename T>
 hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
                        hipStream_t s);
+// LU-form block-tridiagonal factorization / sweeps, one workgroup per chain
+template <typename T>
+bool btri_lu_fused_ok(int bs);
+template <typename T>
+hipError_t btri_lu_factor_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                                int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis,
+                                int* info, hipStream_t s);
+template <typename T>
+hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                               int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
+                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s);
 // block-Thomas factorization / sweeps with explicit diagonal inverses
 template <typename T>
 bool btri_inv_fused_ok(int bs);

@@ -5,15 +5,17 @@ Storage layout is the reference's Tdata: (b, m + 2(m-1) + m//bs + 1, bs, bs)
 = [diag (m) | sub (m-1) | sup (m-1) | pivot blocks] (block_tridiag_solver_dense.py
 :226-257, 291-330), kept on the device.
 
-form='inv' (default): block-Thomas factorization with explicit diagonal
-inverses, diag_i <- S_i^{-1} with S_i = D_i - sub_{i-1} G_{i-1}, sup_i <- G_i =
+form='lu' (default): the reference's block LU, diag_i <- packed LU of
+D_i - Lfac_i Ufac_i, sub_i <- Lfac = sub_i U_i^{-1}, sup_i <- Ufac =
+L_i^{-1} P_i^T sup_i (block_tridiag_solver_dense.py:152-177), pivots packed
+into the pivot blocks as values (1-based).  Chains whose blocks fit in LDS
+are factored and swept by one workgroup each (slab_btri_factor /
+slab_btri_solve); backward stable, which the FP32 (mixed-precision) path
+needs.
+form='inv': block-Thomas factorization with explicit diagonal inverses,
+diag_i <- S_i^{-1} with S_i = D_i - sub_{i-1} G_{i-1}, sup_i <- G_i =
 S_i^{-1} sup_i, sub_i unchanged (slab_btri_inv_factor / slab_btri_inv_solve):
-every sweep step is a matrix product, and blocks that fit in LDS run as one
-workgroup per chain.
-form='lu': the reference's block LU, diag_i <- packed LU of D_i - Lfac_i
-Ufac_i, sub_i <- Lfac = sub_i U_i^{-1}, sup_i <- Ufac = L_i^{-1} P_i^T sup_i
-(block_tridiag_solver_dense.py:152-177), pivots packed into the pivot blocks
-as values (1-based); needed for transposed solves.
+every sweep step is a matrix product; about one digit less accurate.
 """
 import numpy as np
 import torch
@@ -42,7 +44,7 @@ def unpack_pivs(b, m, bs, Tdata, device=None):
 
 
 class Block_Tridiagonal:
-    def __init__(self, b, m, bs, form="inv"):
+    def __init__(self, b, m, bs, form="lu"):
         if form not in ("inv", "lu"):
             raise ValueError("form must be 'inv' or 'lu'")
         self.b, self.m, self.bs = b, m, bs
