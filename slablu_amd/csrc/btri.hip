@@ -434,22 +434,18 @@ __global__ __launch_bounds__(256) void k_getrs_small(int batch, int n, int nrhs,
           }
         }
         __syncthreads();
-        // rows outside the block: below it (lower) or above it (upper)
+        // rows outside the block: below it (lower) or above it (upper); one
+        // wavefront per row, lane j reads LU(r, d0+j) (coalesced)
         const int r0 = lower ? d0 + tb : 0;
         const int r1 = lower ? n : d0;
-        for (int r = r0 + tid; r < r1; r += 256) {
-          T acc[8];
+        for (int r = r0 + wave; r < r1; r += 4) {
+          const T l = (lane < tb) ? *LU.at(b, r, d0 + lane) : T(0);
+          for (int c = 0; c < nrhs; ++c) {
+            T v = (lane < tb) ? l * X[(d0 + lane) * nrhs + c] : T(0);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) acc[c] = T(0);
-          for (int j = 0; j < tb; ++j) {
-            const T l = *LU.at(b, r, d0 + j);
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-              if (c < nrhs) acc[c] += l * X[(d0 + j) * nrhs + c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) X[r * nrhs + c] -= v;
           }
-#pragma unroll
-          for (int c = 0; c < 8; ++c)
-            if (c < nrhs) X[r * nrhs + c] -= acc[c];
         }
         __syncthreads();
       }
@@ -499,6 +495,49 @@ __device__ void tri_solve_lds(At at, int n, bool upper, bool unit, T* X, int nc,
           if (!unit && lane == j) x = x / Dg[j * 65 + j];
           const T xj = __shfl(x, j);
           if (lane < j) x -= Dg[lane * 65 + j] * xj;
+        }
+      }
+      if (lane < tb) X[(d0 + lane) * nc + c] = x;
+    }
+    __syncthreads();
+    const int r0 = upper ? 0 : d0 + tb;
+    const int r1 = upper ? d0 : n;
+    const int nrow = r1 - r0;
+    for (int e = tid; e < nrow * nc; e += 256) {
+      const int rr = e / nc, c = e - rr * nc;
+      const int r = r0 + rr;
+      T acc = T(0);
+      for (int j = 0; j < tb; ++j) acc += at(r, d0 + j) * X[(d0 + j) * nc + c];
+      X[r * nc + c] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Same as tri_solve_lds for a triangular matrix already resident in LDS
+// (`at` reads LDS): no staging of the diagonal blocks.
+template <typename T, typename At>
+__device__ void tri_solve_direct(At at, int n, bool upper, bool unit, T* X, int nc) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + 63) / 64;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = upper ? nblk - 1 - bi : bi;
+    const int d0 = blk * 64;
+    const int tb = (n - d0) < 64 ? (n - d0) : 64;
+    const int rl = d0 + (lane < tb ? lane : 0);
+    for (int c = wave; c < nc; c += 4) {
+      T x = (lane < tb) ? X[(d0 + lane) * nc + c] : T(0);
+      if (!upper) {
+        for (int j = 0; j < tb; ++j) {
+          if (!unit && lane == j) x = x / at(d0 + j, d0 + j);
+          const T xj = __shfl(x, j);
+          if (lane > j && lane < tb) x -= at(rl, d0 + j) * xj;
+        }
+      } else {
+        for (int j = tb - 1; j >= 0; --j) {
+          if (!unit && lane == j) x = x / at(d0 + j, d0 + j);
+          const T xj = __shfl(x, j);
+          if (lane < j) x -= at(rl, d0 + j) * xj;
         }
       }
       if (lane < tb) X[(d0 + lane) * nc + c] = x;
@@ -667,7 +706,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, B
           }
           __syncthreads();
           apply_piv_lds<T>(piv, bs, X, nc);
-          tri_solve_lds<T>(Lat, bs, false, true, X, nc, TB);
+          tri_solve_direct<T>(Lat, bs, false, true, X, nc);
           for (int e = tid; e < bs * nc; e += 256) {
             const int r = e / nc, c = e - r * nc;
             Ui[(int64_t)r * bs + c0 + c] = X[e];
@@ -679,7 +718,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, B
             X[e] = Li[(int64_t)(c0 + c) * bs + r];
           }
           __syncthreads();
-          tri_solve_lds<T>(UTat, bs, false, false, X, nc, TB);
+          tri_solve_direct<T>(UTat, bs, false, false, X, nc);
           for (int e = tid; e < bs * nc; e += 256) {
             const int r = e / nc, c = e - r * nc;
             Li[(int64_t)(c0 + c) * bs + r] = X[e];
@@ -693,47 +732,75 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, B
   }
 }
 
+// Stage the bs x bs row-major block src into LDS Mt (row stride bs+1).
+template <typename T>
+__device__ inline void stage_block(T* Mt, const T* src, int bs) {
+  const int ld = bs + 1;
+  for (int e = threadIdx.x; e < bs * bs; e += 256) {
+    const int r = e / bs, c = e - r * bs;
+    Mt[r * ld + c] = src[e];
+  }
+  __syncthreads();
+}
+
+// Xc[r][c] -= sum_k Mt[r][k] Y[k][c] (all operands in LDS).
+template <typename T, int RPT>
+__device__ inline void sub_mat_vec(const T* Mt, const T* Y, T* Xc, int bs, int nr) {
+  const int tid = threadIdx.x;
+  const int tc = tid % nr, tr = tid / nr, RG = 256 / nr;
+  const int ld = bs + 1;
+  T acc[RPT];
+#pragma unroll
+  for (int x = 0; x < RPT; ++x) acc[x] = T(0);
+  for (int k = 0; k < bs; ++k) {
+    const T y = Y[k * nr + tc];
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) {
+      const int r = tr + RG * x;
+      if (r < bs) acc[x] += Mt[r * ld + k] * y;
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < RPT; ++x) {
+    const int r = tr + RG * x;
+    if (r < bs) Xc[r * nr + tc] -= acc[x];
+  }
+  __syncthreads();
+}
+
 template <typename T, int RPT>
 __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv,
                                                        int64_t pbs, int64_t pis, T* f, int64_t fbs, int64_t ldf,
                                                        int nrhs, int NR) {
   extern __shared__ unsigned char smem_raw[];
-  T* Xc = reinterpret_cast<T*>(smem_raw);  // bs x NR current block
+  T* Mt = reinterpret_cast<T*>(smem_raw);  // bs x (bs+1): one block at a time
+  T* Xc = Mt + bs * (bs + 1);              // bs x NR current block
   T* Y = Xc + bs * NR;                     // bs x NR previous block
-  T* TA = Y + bs * NR;                     // bs x (KC+1)
-  T* Dg = TA + bs * (KC + 1);              // 64 x 65
-  int* P = reinterpret_cast<int*>(Dg + 64 * 65);
+  int* P = reinterpret_cast<int*>(Y + bs * NR);
   const int tid = threadIdx.x;
   const int nr = NR;
-  const int tc = tid % nr, tr = tid / nr, RG = 256 / nr;
+  const int ld = bs + 1;
   const int c0 = blockIdx.y * NR;
   const int ncols = (nrhs - c0) < NR ? (nrhs - c0) : NR;
   for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
     T* fb = f + b * fbs;
     auto F = [&](int i, int r, int c) -> T* { return fb + (int64_t)(i * bs + r) * ldf + c0 + c; };
+    auto Mat = [&](int r, int k) -> T { return Mt[r * ld + k]; };
     for (int i = 0; i < m; ++i) {
       for (int e = tid; e < bs * nr; e += 256) {
         const int r = e / nr, c = e - r * nr;
         Xc[e] = (c < ncols) ? *F(i, r, c) : T(0);
       }
       for (int r = tid; r < bs; r += 256) P[r] = ipiv[b * pbs + (int64_t)i * pis + r] - 1;
-      __syncthreads();
       if (i > 0) {
-        T acc[RPT];
-#pragma unroll
-        for (int x = 0; x < RPT; ++x) acc[x] = T(0);
-        mv_global_lds<T, RPT>(a.sub + b * a.lbs + (int64_t)(i - 1) * a.lis, Y, bs, nr, TA, acc);
-#pragma unroll
-        for (int x = 0; x < RPT; ++x) {
-          const int r = tr + RG * x;
-          if (r < bs) Xc[r * nr + tc] -= acc[x];
-        }
+        stage_block<T>(Mt, a.sub + b * a.lbs + (int64_t)(i - 1) * a.lis, bs);
+        sub_mat_vec<T, RPT>(Mt, Y, Xc, bs, nr);
+      } else {
         __syncthreads();
       }
       apply_piv_lds<T>(P, bs, Xc, nr);
-      const T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
-      auto Lat = [&](int r, int k) -> T { return Di[(int64_t)r * bs + k]; };
-      tri_solve_lds<T>(Lat, bs, false, true, Xc, nr, Dg);
+      stage_block<T>(Mt, a.diag + b * a.dbs + (int64_t)i * a.dis, bs);
+      tri_solve_direct<T>(Mat, bs, false, true, Xc, nr);
       for (int e = tid; e < bs * nr; e += 256) {
         const int r = e / nr, c = e - r * nr;
         Y[e] = Xc[e];
@@ -747,24 +814,13 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
           const int r = e / nr, c = e - r * nr;
           Xc[e] = (c < ncols) ? *F(i, r, c) : T(0);
         }
-        __syncthreads();
-        T acc[RPT];
-#pragma unroll
-        for (int x = 0; x < RPT; ++x) acc[x] = T(0);
-        mv_global_lds<T, RPT>(a.sup + b * a.ubs + (int64_t)i * a.uis, Y, bs, nr, TA, acc);
-#pragma unroll
-        for (int x = 0; x < RPT; ++x) {
-          const int r = tr + RG * x;
-          if (r < bs) Xc[r * nr + tc] -= acc[x];
-        }
-        __syncthreads();
+        stage_block<T>(Mt, a.sup + b * a.ubs + (int64_t)i * a.uis, bs);
+        sub_mat_vec<T, RPT>(Mt, Y, Xc, bs, nr);
       } else {
         for (int e = tid; e < bs * nr; e += 256) Xc[e] = Y[e];
-        __syncthreads();
       }
-      const T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
-      auto Uat = [&](int r, int k) -> T { return Di[(int64_t)r * bs + k]; };
-      tri_solve_lds<T>(Uat, bs, true, false, Xc, nr, Dg);
+      stage_block<T>(Mt, a.diag + b * a.dbs + (int64_t)i * a.dis, bs);
+      tri_solve_direct<T>(Mat, bs, true, false, Xc, nr);
       for (int e = tid; e < bs * nr; e += 256) {
         const int r = e / nr, c = e - r * nr;
         Y[e] = Xc[e];
@@ -783,7 +839,7 @@ size_t lu_factor_lds(int bs) {
 
 template <typename T>
 size_t lu_solve_lds(int bs, int nr) {
-  return sizeof(T) * (2 * (size_t)bs * nr + (size_t)bs * (KC + 1) + 64 * 65) + sizeof(int) * bs;
+  return sizeof(T) * ((size_t)bs * (bs + 1) + 2 * (size_t)bs * nr) + sizeof(int) * bs;
 }
 
 template <typename T>

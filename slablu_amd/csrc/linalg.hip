@@ -110,9 +110,58 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
   }
 }
 
+// Few right-hand-side columns (n <= 8) with a row-major A: one wavefront per
+// output row, lanes striding k (coalesced rows of A), shuffle reduction.
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_gemv(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta,
+                                              View<T> C) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    for (int row = blockIdx.x * 4 + wave; row < m; row += gridDim.x * 4) {
+      T acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = T(0);
+      const T* arow = A.at(b, row, 0);
+      for (int kk = lane; kk < k; kk += 64) {
+        const T a = arow[kk];
+        const T* brow = B.at(b, kk, 0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c < n) acc[c] += a * brow[c * B.cs];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off);
+      }
+      if (lane < n) {
+        T v = T(0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c == lane) v = acc[c];
+        T* cp = C.at(b, row, lane);
+        T out = alpha * v;
+        if (beta != T(0)) out += beta * (*cp);
+        *cp = out;
+      }
+    }
+  }
+}
+
 template <typename T>
 hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  if (n <= 8 && A.cs == 1 && k >= 32) {
+    const int gx = (m + 3) / 4 < 8192 ? (m + 3) / 4 : 8192;
+    dim3 grid(gx, grid_batch(batch));
+    if (n <= 1)
+      hipLaunchKernelGGL((k_gemv<T, 1>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    else if (n <= 4)
+      hipLaunchKernelGGL((k_gemv<T, 4>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    else
+      hipLaunchKernelGGL((k_gemv<T, 8>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    return hipGetLastError();
+  }
   if (k <= 0) {
     // C = beta * C: run the kernel with a zero-length K loop.
     k = 0;

@@ -66,7 +66,7 @@ def _blockdiag(blocks):
 
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
-                 device=None, refine_tol=1e-11, refine_maxit=8):
+                 device=None, refine_tol=1e-9, refine_maxit=12):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
         if periodic_bc:

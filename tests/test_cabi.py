@@ -1,0 +1,47 @@
+"""The C-ABI library builds, loads on a host without a GPU and exports every
+entry point declared in include/slab_hip.h (no compute calls here)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "slab_hip.h")).read()
+    return sorted(set(re.findall(r"^int\s+(slab_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ("slab_gemm", "slab_getrf", "slab_btri_factor", "slab_btri_solve", "slab_btri_inv_factor",
+                 "slab_btri_inv_solve", "slab_leaf_assemble"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from slablu_amd import hip
+    if not os.path.exists(hip.LIB_PATH):
+        from slablu_amd import _build
+        _build.build()
+    lib = ctypes.CDLL(hip.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.slab_version() >= 100
+
+
+def test_bad_arguments_are_rejected_without_a_gpu():
+    """Argument validation happens before any launch: negative sizes and
+    inconsistent LU shapes return -1."""
+    from slablu_amd import hip
+    if not os.path.exists(hip.LIB_PATH):
+        from slablu_amd import _build
+        _build.build()
+    lib = ctypes.CDLL(hip.LIB_PATH)
+    lib.slab_getrf.restype = ctypes.c_int
+    args = [ctypes.c_int(0), ctypes.c_int(1), ctypes.c_int(10), ctypes.c_int(10), ctypes.c_int(11),
+            ctypes.c_int(10), ctypes.c_void_p(0), ctypes.c_int64(10), ctypes.c_int64(100), ctypes.c_void_p(0),
+            ctypes.c_int64(10), ctypes.c_void_p(0), ctypes.c_void_p(0)]
+    assert lib.slab_getrf(*args) == -1  # nelim > npiv
+    assert lib.slab_btri_inv_fused(0, 100) == 1 and lib.slab_btri_inv_fused(0, 1000) == 0
+    assert lib.slab_btri_inv_fused(7, 10) == -1
