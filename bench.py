@@ -73,7 +73,7 @@ def run_path(dom, uu_dir, steps, warmup, leaf_events):
             ts.append(t1 - t0)
             tf.append(info["toc_build"])
             tsol.append(toc)
-            for k in ("toc_DtN", "toc_Aloc", "toc_Ared", "toc_Tred"):
+            for k in ("toc_DtN", "toc_Aloc", "toc_Ared", "toc_Tred", "toc_Tform"):
                 stages[k] = stages.get(k, 0.0) + 1e3 * info[k] / steps
             if dom.hps.leaf_event_pair is not None:
                 e0, e1 = dom.hps.leaf_event_pair

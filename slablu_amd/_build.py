@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libslabhip.so")
-SOURCES = ["linalg.hip", "leaf.hip", "btri.hip", "api.hip"]
+SOURCES = ["linalg.hip", "leaf.hip", "btri_inv.hip", "btri_lu_factor.hip", "btri_lu_solve.hip", "getrs_small.hip", "api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -1,0 +1,506 @@
+// Device building blocks shared by the block-tridiagonal and small-solve
+// kernels: LDS-resident dense operations for one 256-thread workgroup
+// (matrix products staged through LDS, LU with partial pivoting, triangular
+// solves, pivot permutations) and the LDS size rules of the fused kernels.
+#pragma once
+
+#include "slab_internal.h"
+
+#include <climits>
+
+namespace slab {
+
+namespace {
+
+constexpr int KC = 16;  // k-chunk staged through LDS in the matrix products
+
+template <typename T>
+__device__ inline T absv(T v) { return v < T(0) ? -v : v; }
+
+template <typename T>
+struct BtriArgs {
+  T* diag; int64_t dbs, dis;
+  T* sub; int64_t lbs, lis;
+  T* sup; int64_t ubs, uis;
+};
+
+// acc[a][q] += sum_k A(r_a, k) B(k, c_q) for bs x bs row-major A and B in
+// global memory; thread (ty, tx) of a 16 x 16 arrangement owns rows
+// ty + 16a and columns tx + 16q (a, q < RT).  K runs in KC-wide chunks
+// staged through LDS (TA: 16RT x (KC+1), TB: KC x (16RT+1)), entries past
+// bs staged as zeros so the inner loop has no bounds tests, and the next
+// chunk is loaded into registers while the current one is multiplied.
+template <typename T, int RT>
+__device__ void mm_global_global(const T* A, const T* B, int bs, T* TA, T* TB, T (&acc)[RT][RT]) {
+  constexpr int BP = 16 * RT;
+  constexpr int NE = (BP * KC + 255) / 256;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int bm1 = bs - 1;
+  T va[NE], vb[NE];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      const int ra = e / KC, ka = k0 + e % KC;
+      const T a = A[(int64_t)(ra < bm1 ? ra : bm1) * bs + (ka < bm1 ? ka : bm1)];
+      va[u] = (ra < bs && ka < bs) ? a : T(0);
+      const int kb = k0 + e / BP, cb = e % BP;
+      const T v = B[(int64_t)(kb < bm1 ? kb : bm1) * bs + (cb < bm1 ? cb : bm1)];
+      vb[u] = (kb < bs && cb < bs) ? v : T(0);
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < bs; k0 += KC) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      if (e < BP * KC) {
+        TA[(e / KC) * (KC + 1) + e % KC] = va[u];
+        TB[(e / BP) * (BP + 1) + e % BP] = vb[u];
+      }
+    }
+    __syncthreads();
+    if (k0 + KC < bs) load(k0 + KC);
+#pragma unroll 4
+    for (int kk = 0; kk < KC; ++kk) {
+      T av[RT], bv[RT];
+#pragma unroll
+      for (int a = 0; a < RT; ++a) av[a] = TA[(ty + 16 * a) * (KC + 1) + kk];
+#pragma unroll
+      for (int q = 0; q < RT; ++q) bv[q] = TB[kk * (BP + 1) + tx + 16 * q];
+#pragma unroll
+      for (int a = 0; a < RT; ++a)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) acc[a][q] += av[a] * bv[q];
+    }
+    __syncthreads();
+  }
+}
+
+// acc[a][q] += sum_k S(r_a, k) B(k, c_q): S in LDS (row stride bs+1), B global.
+template <typename T, int RT>
+__device__ void mm_lds_global(const T* S, const T* B, int bs, T* TB, T (&acc)[RT][RT]) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int ld = bs + 1;
+  for (int k0 = 0; k0 < bs; k0 += KC) {
+    for (int e = tid; e < KC * bs; e += 256) {
+      const int kk = e / bs, c = e - kk * bs;
+      TB[kk * (bs + 1) + c] = (k0 + kk < bs) ? B[(int64_t)(k0 + kk) * bs + c] : T(0);
+    }
+    __syncthreads();
+    const int kn = (bs - k0) < KC ? (bs - k0) : KC;
+    for (int kk = 0; kk < kn; ++kk) {
+      T av[RT], bv[RT];
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+        const int r = ty + 16 * a;
+        av[a] = (r < bs) ? S[r * ld + k0 + kk] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int c = tx + 16 * q;
+        bv[q] = (c < bs) ? TB[kk * (bs + 1) + c] : T(0);
+      }
+#pragma unroll
+      for (int a = 0; a < RT; ++a)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) acc[a][q] += av[a] * bv[q];
+    }
+    __syncthreads();
+  }
+}
+
+// Broadcast lane j's value of v to the wavefront (j uniform).
+__device__ inline float rdlane(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double rdlane(double v, int j) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// e / d for 0 <= e < 2^21 with rd = 1.0f / d: (e + 0.5) / d lies at least
+// 0.5 / d from an integer and the two single-precision roundings move it by
+// less than 2^-22 (e + 0.5) / d, so the truncation is exact.
+__device__ inline int idiv(int e, float rd) { return (int)(((float)e + 0.5f) * rd); }
+
+// Q[r] = the row of X that lands in row r when the interchanges row k <->
+// row P[k] (P[k] >= k, 0-based) are applied for k = 0..n-1 in order; thread
+// v follows where row v ends up, so no step is sequential across threads.
+__device__ inline void piv_to_perm(const int* P, int n, int* Q) {
+  for (int v = threadIdx.x; v < n; v += 256) {
+    int pos = v;
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) {
+      const int pk = P[k];
+      pos = (pos == k) ? pk : ((pos == pk) ? k : pos);
+    }
+    Q[pos] = v;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// LU form (backward stable; the reference's block LU): diag_i <- packed LU of
+// D_i - Lfac_{i-1} Ufac_{i-1}, sub_i <- Lfac_i = sub_i U_i^{-1}, sup_i <-
+// Ufac_i = L_i^{-1} P_i^T sup_i, ipiv (b, m, bs) 1-based.
+// ---------------------------------------------------------------------------
+
+// Triangular solve M X = X in LDS for nc <= NC <= 8 columns (X row stride
+// ldx): wavefront w substitutes columns w, w + 4, ... on each 64-row
+// diagonal block with lane = row and readlane broadcasts (non-unit diagonals
+// through their reciprocals), then all threads update the rows beyond the
+// block, one thread per row.  M(r, k) is read through `at` (LDS).  Every LDS
+// read in the loops is unconditional (clamped indices, masked results) so
+// that reads are issued back to back.
+template <typename T, bool UPPER, bool UNIT, int NC, typename At>
+__device__ void tri_solve_wave(At at, int n, T* X, int ldx, int nc) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + 63) >> 6;
+  const int cm1 = nc - 1;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = UPPER ? nblk - 1 - bi : bi;
+    const int d0 = blk << 6;
+    const int tb = (n - d0) < 64 ? (n - d0) : 64;
+    const bool act = lane < tb;
+    const int rl = d0 + (act ? lane : tb - 1);
+    for (int c = wave; c < nc; c += 4) {
+      T x = X[rl * ldx + c];
+      const T rinv = UNIT ? T(1) : T(1) / at(rl, rl);
+      if (UPPER) {
+#pragma unroll 8
+        for (int j = tb - 1; j >= 0; --j) {
+          const T a = at(rl, d0 + j);
+          const T xj = UNIT ? rdlane(x, j) : rdlane(x, j) * rdlane(rinv, j);
+          x = (lane == j) ? xj : ((lane < j) ? x - a * xj : x);
+        }
+      } else {
+#pragma unroll 8
+        for (int j = 0; j < tb; ++j) {
+          const T a = at(rl, d0 + j);
+          const T xj = UNIT ? rdlane(x, j) : rdlane(x, j) * rdlane(rinv, j);
+          x = (lane == j) ? xj : ((lane > j) ? x - a * xj : x);
+        }
+      }
+      if (act) X[rl * ldx + c] = x;
+    }
+    __syncthreads();
+    const int r0 = UPPER ? 0 : d0 + tb;
+    const int r1 = UPPER ? d0 : n;
+    for (int r = r0 + tid; r < r1; r += 256) {
+      T acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = T(0);
+#pragma unroll 8
+      for (int j = 0; j < tb; ++j) {
+        const T a = at(r, d0 + j);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a * X[(d0 + j) * ldx + (c < cm1 ? c : cm1)];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < nc) X[r * ldx + c] -= acc[c];
+    }
+    __syncthreads();
+  }
+}
+
+// Triangular solve M X = X in LDS for many columns: each 16-row diagonal
+// block is solved by one thread per column in registers, then the rows
+// beyond it are updated by one thread per (column, row group) holding the
+// block's 16 solved values.  Reads past the last row are clamped to it;
+// the values they produce only reach rows that are never stored (lower) or
+// are multiplied by zeros (upper).
+template <typename T, bool UPPER, bool UNIT, typename At>
+__device__ void tri_solve_cols(At at, int n, T* X, int ldx, int nc) {
+  constexpr int DB = 16;
+  const int tid = threadIdx.x;
+  const int nblk = (n + DB - 1) / DB;
+  const int nm1 = n - 1;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int blk = UPPER ? nblk - 1 - bi : bi;
+    const int d0 = blk * DB;
+    const int tb = (n - d0) < DB ? (n - d0) : DB;
+    for (int c = tid; c < nc; c += 256) {
+      T x[DB];
+#pragma unroll
+      for (int j = 0; j < DB; ++j) {
+        const T v = X[(d0 + (j < tb ? j : tb - 1)) * ldx + c];
+        x[j] = (j < tb) ? v : T(0);
+      }
+      if (UPPER) {
+#pragma unroll
+        for (int j = DB - 1; j >= 0; --j) {
+          const int jj = (d0 + j) < nm1 ? (d0 + j) : nm1;
+          T xj = UNIT ? x[j] : x[j] / at(jj, jj);
+          xj = (j < tb) ? xj : T(0);
+          x[j] = xj;
+#pragma unroll
+          for (int r = 0; r < j; ++r) x[r] -= at((d0 + r) < nm1 ? (d0 + r) : nm1, jj) * xj;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < DB; ++j) {
+          const int jj = (d0 + j) < nm1 ? (d0 + j) : nm1;
+          const T xj = UNIT ? x[j] : x[j] / at(jj, jj);
+          x[j] = xj;
+#pragma unroll
+          for (int r = j + 1; r < DB; ++r) x[r] -= at((d0 + r) < nm1 ? (d0 + r) : nm1, jj) * xj;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < DB; ++j)
+        if (j < tb) X[(d0 + j) * ldx + c] = x[j];
+    }
+    __syncthreads();
+    const int r0 = UPPER ? 0 : d0 + tb;
+    const int r1 = UPPER ? d0 : n;
+    if (r1 > r0) {
+      for (int cc = 0; cc < nc; cc += 256) {
+        const int nch = (nc - cc) < 256 ? (nc - cc) : 256;
+        const int G = 256 / nch;
+        const int c = cc + tid % nch, g = tid / nch;
+        if (g < G) {
+          T xc[DB];
+#pragma unroll
+          for (int j = 0; j < DB; ++j) {
+            const T v = X[(d0 + (j < tb ? j : tb - 1)) * ldx + c];
+            xc[j] = (j < tb) ? v : T(0);
+          }
+          for (int r = r0 + g; r < r1; r += G) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < DB; ++j) acc += at(r, (d0 + j) < nm1 ? (d0 + j) : nm1) * xc[j];
+            X[r * ldx + c] -= acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool UPPER, bool UNIT, typename At>
+__device__ inline void tri_solve(At at, int n, T* X, int ldx, int nc) {
+  if (nc == 1)
+    tri_solve_wave<T, UPPER, UNIT, 1>(at, n, X, ldx, nc);
+  else if (nc == 2)
+    tri_solve_wave<T, UPPER, UNIT, 2>(at, n, X, ldx, nc);
+  else if (nc <= 4)
+    tri_solve_wave<T, UPPER, UNIT, 4>(at, n, X, ldx, nc);
+  else if (nc <= 8)
+    tri_solve_wave<T, UPPER, UNIT, 8>(at, n, X, ldx, nc);
+  else
+    tri_solve_cols<T, UPPER, UNIT>(at, n, X, ldx, nc);
+}
+
+// In-place LU with partial pivoting of the n x n matrix S in LDS (row stride
+// ld >= 16RT + 1, n <= 16RT): thread (ty, tx) of a 16 x 16 arrangement owns
+// rows ty + 16a and columns tx + 16q of the padded 16RT x 16RT tile and
+// updates it with no bounds tests (padding rows/columns are masked out
+// through zero multipliers and hold garbage on return).  Per column: pivot
+// search (wavefront reductions, every thread then picks the winner of the
+// four wavefronts), row interchange, rank-1 update; the multipliers of
+// column k are written back during the pivot search of column k + 1, so
+// each column costs three barriers.  piv[k] is 0-based.
+template <typename T, int RT>
+__device__ void lu_lds(T* S, int n, int ld, int* piv, T* red_v, int* red_i, int* info, int64_t b) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int nm1 = n - 1;
+  T colk[RT];
+  for (int k = 0; k < n; ++k) {
+    if (k > 0 && tx == ((k - 1) & 15)) {
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+        const int r = ty + 16 * a;
+        if (r > k - 1 && r < n) S[r * ld + k - 1] = colk[a];
+      }
+    }
+    T best = T(-1);
+    int bi = INT_MAX;
+    for (int r0 = k; r0 < n; r0 += 256) {
+      const int r = r0 + tid;
+      const T v = absv(S[(r < nm1 ? r : nm1) * ld + k]);
+      const bool ok = r < n && v > best;
+      best = ok ? v : best;
+      bi = ok ? r : bi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      const bool take = ov > best || (ov == best && oi < bi);
+      best = take ? ov : best;
+      bi = take ? oi : bi;
+    }
+    if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+    __syncthreads();
+    T bv = red_v[0];
+    int p = red_i[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+      const T ov = red_v[q];
+      const int oi = red_i[q];
+      const bool take = ov > bv || (ov == bv && oi < p);
+      bv = take ? ov : bv;
+      p = take ? oi : p;
+    }
+    if (p < k || p >= n) p = k;
+    if (tid == 0) piv[k] = p;
+    if (p != k) {
+      for (int c = tid; c < n; c += 256) {
+        const T t = S[k * ld + c];
+        S[k * ld + c] = S[p * ld + c];
+        S[p * ld + c] = t;
+      }
+    }
+    __syncthreads();
+    const T pv = S[k * ld + k];
+    const bool nz = pv != T(0);
+    if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k + 1;
+    T rowk[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      const int c = tx + 16 * q;
+      const T v = S[k * ld + c];
+      rowk[q] = (c > k && c < n) ? v : T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+      const int r = ty + 16 * a;
+      const T v = S[r * ld + k];
+      colk[a] = (r > k && r < n) ? (nz ? v / pv : v) : T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int q = 0; q < RT; ++q) S[(ty + 16 * a) * ld + tx + 16 * q] -= colk[a] * rowk[q];
+    __syncthreads();
+  }
+}
+
+// Stage the bs x bs row-major block src into LDS Mt (row stride ld), with
+// eight loads in flight per thread.
+template <typename T>
+__device__ inline void stage_block(T* Mt, const T* src, int bs, float rbs, int ld) {
+  const int nn = bs * bs;
+  for (int e0 = threadIdx.x; e0 < nn; e0 += 256 * 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      v[u] = src[e < nn ? e : nn - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      const int r = idiv(e, rbs);
+      if (e < nn) Mt[r * ld + e - r * bs] = v[u];
+    }
+  }
+  __syncthreads();
+}
+
+// Xc[r][c] -= sum_k Mt[r][k] Y[k][c] (all operands in LDS, Mt row stride
+// bs+1, X row stride ldx); thread (tr, tc) owns column tc and rows
+// tr + (256/nr) x, read from clamped rows so the k loop has no tests.
+template <typename T, int RPT>
+__device__ inline void sub_mat_vec(const T* Mt, const T* Y, T* Xc, int bs, int nr, int ldx) {
+  const int tid = threadIdx.x;
+  const int tc = tid % nr, tr = tid / nr, RG = 256 / nr;
+  const int ld = bs + 1;
+  int off[RPT];
+  T acc[RPT];
+#pragma unroll
+  for (int x = 0; x < RPT; ++x) {
+    const int r = tr + RG * x;
+    off[x] = (r < bs ? r : bs - 1) * ld;
+    acc[x] = T(0);
+  }
+  const T* yp = Y + tc;
+#pragma unroll 4
+  for (int k = 0; k < bs; ++k) {
+    const T y = yp[k * ldx];
+#pragma unroll
+    for (int x = 0; x < RPT; ++x) acc[x] += Mt[off[x] + k] * y;
+  }
+#pragma unroll
+  for (int x = 0; x < RPT; ++x) {
+    const int r = tr + RG * x;
+    if (r < bs) Xc[r * ldx + tc] -= acc[x];
+  }
+  __syncthreads();
+}
+
+// Single-column Xc -= Mt y: for bs <= 128 two threads per row split k (the
+// upper half's partial sums pass through W, bs elements of LDS).
+template <typename T>
+__device__ inline void sub_mat_vec1(const T* Mt, const T* Y, T* Xc, int bs, T* W) {
+  const int tid = threadIdx.x, ld = bs + 1;
+  if (bs <= 128) {
+    const int r = tid & 127, h = tid >> 7;
+    const int rc = (r < bs ? r : bs - 1) * ld;
+    const int kh = (bs + 1) >> 1;
+    const int k0 = h * kh, k1 = (k0 + kh) < bs ? (k0 + kh) : bs;
+    T acc = T(0);
+#pragma unroll 8
+    for (int k = k0; k < k1; ++k) acc += Mt[rc + k] * Y[k];
+    if (h == 1 && r < bs) W[r] = acc;
+    __syncthreads();
+    if (h == 0 && r < bs) Xc[r] -= acc + W[r];
+  } else {
+    for (int r = tid; r < bs; r += 256) {
+      T acc = T(0);
+#pragma unroll 8
+      for (int k = 0; k < bs; ++k) acc += Mt[r * ld + k] * Y[k];
+      Xc[r] -= acc;
+    }
+  }
+  __syncthreads();
+}
+
+constexpr size_t kLdsMax = 160 * 1024;
+
+// 16 x 16 thread tiles per dimension of a bs x bs block (the RT of the LU
+// and Gauss-Jordan kernels, which pad blocks to 16 RT).
+inline int tile_rt(int bs) { return (bs + 15) / 16; }
+
+template <typename T>
+size_t lu_factor_lds(int bs, int xc) {
+  const size_t bp = 16 * (size_t)tile_rt(bs);
+  return sizeof(T) * (bp * (bp + 1) + (size_t)bs * (xc + 1) + bp * (KC + 1) + (size_t)KC * (bp + 1)) +
+         2 * sizeof(int) * bs;
+}
+
+// widest right-hand-side chunk of the factor kernel that fits in LDS
+template <typename T>
+int lu_factor_xc(int bs) {
+  int xc = bs;
+  while (xc > 8 && lu_factor_lds<T>(bs, xc) > kLdsMax) xc -= 8;
+  return xc;
+}
+
+template <typename T>
+size_t lu_solve_lds(int bs, int nr) {
+  const size_t ldx = nr == 1 ? 1 : nr + 1;
+  return sizeof(T) * ((size_t)bs * (bs + 1) + 2 * (size_t)bs * ldx + (size_t)bs) + 2 * sizeof(int) * bs;
+}
+
+// LDS of the Gauss-Jordan factor kernel instantiated with RT tiles
+template <typename T>
+size_t factor_lds(int bs, int rt) {
+  const size_t bp = 16 * (size_t)rt;
+  return sizeof(T) * ((size_t)bs * (bs + 1) + bp * (KC + 1) + (size_t)KC * (bp + 1)) + sizeof(int) * bs;
+}
+
+template <typename T>
+size_t solve_lds(int bs, int nr) {
+  return sizeof(T) * (2 * (size_t)bs * nr + (size_t)bs * (KC + 1));
+}
+
+}  // namespace
+
+}  // namespace slab

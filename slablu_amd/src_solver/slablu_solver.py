@@ -78,6 +78,8 @@ class Domain_Solver:
         self.buf_constant = buf_constant
         self.precision = precision
         self.refine_tol, self.refine_maxit = refine_tol, refine_maxit
+        # synchronize inside build to split stage timings (bench/profiling only)
+        self.stage_sync = True
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.hps_disc(box_geom, h, p, pdo_op)
         self.hps_panel_split()
@@ -153,9 +155,12 @@ class Domain_Solver:
         sync()
         t["toc_Ared"] = time.time() - tic
         tic = time.time()
+        self._t_tred0 = tic
+        self.t_Tform = float("nan")
         self.build_Tred_solver()
         sync()
         t["toc_Tred"] = time.time() - tic
+        t["toc_Tform"] = self.t_Tform
         toc_build = time.time() - tic0
         mem = (self.hps.leaf["M"].numel() * self.hps.leaf["M"].element_size() + self.Aloc_solver.nbytes +
                self.Ared_solver.nbytes + self.Tred_solver.nbytes) / 1e9
@@ -319,6 +324,9 @@ class Domain_Solver:
         K6[:, ja, :, :, ja, :] += Q.view(Np, n1, 2 * f, 2, f).permute(1, 0, 2, 3, 4)
         Kf = K.view(Np, n1, 2, f, 2 * L).permute(0, 2, 1, 3, 4).reshape(Np, 2 * L, 2 * L)
         del upad, K
+        if self.stage_sync:
+            torch.cuda.synchronize()
+            self.t_Tform = time.time() - self._t_tred0
         # interface s between slab s and s+1
         Aii = self.interface_blocks()
         KR = Kf[:, L:, L:].to(torch.float64)
