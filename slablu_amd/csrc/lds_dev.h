@@ -5,6 +5,7 @@
 #pragma once
 
 #include "slab_internal.h"
+#include "wave_ops.h"
 
 #include <climits>
 
@@ -108,17 +109,6 @@ __device__ void mm_lds_global(const T* S, const T* B, int bs, T* TB, T (&acc)[RT
     }
     __syncthreads();
   }
-}
-
-// Broadcast lane j's value of v to the wavefront (j uniform).
-__device__ inline float rdlane(float v, int j) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ inline double rdlane(double v, int j) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // e / d for 0 <= e < 2^21 with rd = 1.0f / d: (e + 0.5) / d lies at least
@@ -328,26 +318,13 @@ __device__ void lu_lds(T* S, int n, int ld, int* piv, T* red_v, int* red_i, int*
       best = ok ? v : best;
       bi = ok ? r : bi;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const T ov = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      const bool take = ov > best || (ov == best && oi < bi);
-      best = take ? ov : best;
-      bi = take ? oi : bi;
-    }
+    wave_argmax(best, bi);
     if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
     __syncthreads();
     T bv = red_v[0];
     int p = red_i[0];
 #pragma unroll
-    for (int q = 1; q < 4; ++q) {
-      const T ov = red_v[q];
-      const int oi = red_i[q];
-      const bool take = ov > bv || (ov == bv && oi < p);
-      bv = take ? ov : bv;
-      p = take ? oi : p;
-    }
+    for (int q = 1; q < 4; ++q) argmax_merge(bv, p, red_v[q], red_i[q]);
     if (p < k || p >= n) p = k;
     if (tid == 0) piv[k] = p;
     if (p != k) {

@@ -4,6 +4,7 @@
 // Every kernel takes strided views (row, column and batch strides) so that
 // transposes, sub-blocks and broadcast operands need no copies.
 #include "slab_internal.h"
+#include "wave_ops.h"
 
 #include <climits>
 
@@ -57,26 +58,43 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
   const int wm = wave >> 1, wn = wave & 1;
   const bool a_rowmajor = (A.cs == 1);
   const bool b_rowmajor = (B.cs == 1);
+  const int mm1 = m - 1, nm1 = n - 1, km1 = k - 1;
+  constexpr int NU = GBM * GBK / 256;
   for (int b = blockIdx.y; b < batch; b += gridDim.y) {
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    for (int k0 = 0; k0 < k; k0 += GBK) {
-      for (int e = tid; e < GBM * GBK; e += 256) {
+    // operands of the next K chunk travel in registers while the current
+    // chunk is multiplied; out-of-range entries are read clamped and zeroed
+    T ra[NU], rb[NU];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int e = tid + 256 * u;
         int r, kk;
         if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
         const int gr = m0 + r, gk = k0 + kk;
-        As[kk][r] = (gr < m && gk < k) ? *A.at(b, gr, gk) : T(0);
+        const T va = *A.at(b, gr < mm1 ? gr : mm1, gk < km1 ? gk : km1);
+        ra[u] = (gr < m && gk < k) ? va : T(0);
+        int c, kb;
+        if (b_rowmajor) { kb = e / GBN; c = e % GBN; } else { kb = e % GBK; c = e / GBK; }
+        const int gc = n0 + c, gkb = k0 + kb;
+        const T vb = *B.at(b, gkb < km1 ? gkb : km1, gc < nm1 ? gc : nm1);
+        rb[u] = (gc < n && gkb < k) ? vb : T(0);
       }
-      for (int e = tid; e < GBK * GBN; e += 256) {
-        int c, kk;
-        if (b_rowmajor) { kk = e / GBN; c = e % GBN; } else { kk = e % GBK; c = e / GBK; }
-        const int gc = n0 + c, gk = k0 + kk;
-        Bs[kk][c] = (gc < n && gk < k) ? *B.at(b, gk, gc) : T(0);
+    };
+    if (k > 0) load(0);
+    for (int k0 = 0; k0 < k; k0 += GBK) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int e = tid + 256 * u;
+        if (a_rowmajor) As[e % GBK][e / GBK] = ra[u]; else As[e / GBM][e % GBM] = ra[u];
+        if (b_rowmajor) Bs[e / GBN][e % GBN] = rb[u]; else Bs[e % GBK][e / GBK] = rb[u];
       }
       __syncthreads();
+      if (k0 + GBK < k) load(k0 + GBK);
 #pragma unroll
       for (int kb = 0; kb < GBK; kb += 4) {
         T af[2], bf[2];
@@ -91,6 +109,20 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
       }
       __syncthreads();
     }
+    T cv[2][2][4];
+    if (beta != T(0)) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
+            cv[i][j][q] = *C.at(b, row < mm1 ? row : mm1, col < nm1 ? col : nm1);
+          }
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -100,10 +132,9 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
         for (int q = 0; q < 4; ++q) {
           const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
           if (row < m && col < n) {
-            T* cp = C.at(b, row, col);
             T v = alpha * acc[i][j][q];
-            if (beta != T(0)) v += beta * (*cp);
-            *cp = v;
+            if (beta != T(0)) v += beta * cv[i][j][q];
+            *C.at(b, row, col) = v;
           }
         }
       }
@@ -122,12 +153,13 @@ __global__ __launch_bounds__(256) void k_gemv(int batch, int m, int n, int k, T 
 #pragma unroll
       for (int c = 0; c < NC; ++c) acc[c] = T(0);
       const T* arow = A.at(b, row, 0);
+      const int nm1 = n - 1;
+#pragma unroll 4
       for (int kk = lane; kk < k; kk += 64) {
         const T a = arow[kk];
         const T* brow = B.at(b, kk, 0);
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (c < n) acc[c] += a * brow[c * B.cs];
+        for (int c = 0; c < NC; ++c) acc[c] += a * brow[(int64_t)(c < nm1 ? c : nm1) * B.cs];
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -174,51 +206,51 @@ hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T
 
 // ---------------------------------------------------------------------------
 // Triangular solve of a <= 32 x 32 diagonal block: one thread per right-hand
-// side column, the block in LDS, the solution in registers.
+// side column (one wavefront per workgroup), the block in LDS padded with
+// the identity, the column in registers; all loads unconditional (clamped).
 // ---------------------------------------------------------------------------
 constexpr int TB = 32;
 
 template <typename T>
-__global__ __launch_bounds__(128) void k_trsm_diag(int batch, int n, int d0, int tb, int lower, int unit, View<T> A,
-                                                   View<T> B) {
+__global__ __launch_bounds__(64) void k_trsm_diag(int batch, int n, int d0, int tb, int lower, int unit, View<T> A,
+                                                  View<T> B) {
   __shared__ T Ls[TB][TB + 1];
   const int tid = threadIdx.x;
-  const int col = blockIdx.x * 128 + tid;
+  const int col = blockIdx.x * 64 + tid;
+  const int cl = col < n ? col : n - 1;
+  const int tm1 = tb - 1;
   for (int b = blockIdx.y; b < batch; b += gridDim.y) {
-    for (int e = tid; e < TB * TB; e += 128) {
-      const int r = e / TB, c = e % TB;
-      Ls[r][c] = (r < tb && c < tb) ? *A.at(b, d0 + r, d0 + c) : T(0);
+#pragma unroll
+    for (int u = 0; u < TB * TB / 64; ++u) {
+      const int e = tid + 64 * u, r = e / TB, c = e % TB;
+      const T v = *A.at(b, d0 + (r < tm1 ? r : tm1), d0 + (c < tm1 ? c : tm1));
+      Ls[r][c] = (r < tb && c < tb) ? v : ((r == c) ? T(1) : T(0));
     }
     __syncthreads();
-    if (col < n) {
-      T x[TB];
+    T x[TB];
 #pragma unroll
-      for (int r = 0; r < TB; ++r) x[r] = (r < tb) ? *B.at(b, d0 + r, col) : T(0);
-      if (lower) {
+    for (int r = 0; r < TB; ++r) {
+      const T v = *B.at(b, d0 + (r < tm1 ? r : tm1), cl);
+      x[r] = (r < tb) ? v : T(0);
+    }
+    if (lower) {
 #pragma unroll
-        for (int i = 0; i < TB; ++i) {
-          if (i < tb) {
-            T v = x[i];
+      for (int i = 0; i < TB; ++i) {
+        T v = x[i];
 #pragma unroll
-            for (int j = 0; j < TB; ++j)
-              if (j < i) v -= Ls[i][j] * x[j];
-            if (!unit) v = v / Ls[i][i];
-            x[i] = v;
-          }
-        }
-      } else {
-#pragma unroll
-        for (int i = TB - 1; i >= 0; --i) {
-          if (i < tb) {
-            T v = x[i];
-#pragma unroll
-            for (int j = 0; j < TB; ++j)
-              if (j > i && j < tb) v -= Ls[i][j] * x[j];
-            if (!unit) v = v / Ls[i][i];
-            x[i] = v;
-          }
-        }
+        for (int j = 0; j < i; ++j) v -= Ls[i][j] * x[j];
+        x[i] = unit ? v : v / Ls[i][i];
       }
+    } else {
+#pragma unroll
+      for (int i = TB - 1; i >= 0; --i) {
+        T v = x[i];
+#pragma unroll
+        for (int j = i + 1; j < TB; ++j) v -= Ls[i][j] * x[j];
+        x[i] = unit ? v : v / Ls[i][i];
+      }
+    }
+    if (col < n) {
 #pragma unroll
       for (int r = 0; r < TB; ++r)
         if (r < tb) *B.at(b, d0 + r, col) = x[r];
@@ -230,12 +262,12 @@ __global__ __launch_bounds__(128) void k_trsm_diag(int batch, int n, int d0, int
 template <typename T>
 hipError_t trsm_left(int batch, int m, int n, bool lower, bool unit, View<T> A, View<T> B, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
-  dim3 grid(ceil_div(n, 128), grid_batch(batch));
+  dim3 grid(ceil_div(n, 64), grid_batch(batch));
   hipError_t e = hipSuccess;
   if (lower) {
     for (int d0 = 0; d0 < m; d0 += TB) {
       const int tb = (m - d0) < TB ? (m - d0) : TB;
-      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(128), 0, s, batch, n, d0, tb, 1, unit ? 1 : 0, A, B);
+      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(64), 0, s, batch, n, d0, tb, 1, unit ? 1 : 0, A, B);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if (d0 + tb < m) {
         e = gemm<T>(batch, m - d0 - tb, n, tb, T(-1), A.sub(d0 + tb, d0), B.sub(d0, 0), T(1), B.sub(d0 + tb, 0), s);
@@ -246,7 +278,7 @@ hipError_t trsm_left(int batch, int m, int n, bool lower, bool unit, View<T> A, 
     const int last = ((m - 1) / TB) * TB;
     for (int d0 = last; d0 >= 0; d0 -= TB) {
       const int tb = (m - d0) < TB ? (m - d0) : TB;
-      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(128), 0, s, batch, n, d0, tb, 0, unit ? 1 : 0, A, B);
+      hipLaunchKernelGGL(k_trsm_diag<T>, grid, dim3(64), 0, s, batch, n, d0, tb, 0, unit ? 1 : 0, A, B);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if (d0 > 0) {
         e = gemm<T>(batch, d0, n, tb, T(-1), A.sub(0, d0), B.sub(d0, 0), T(1), B, s);
@@ -448,13 +480,348 @@ __global__ __launch_bounds__(256) void k_laswp_panel(int batch, int ncols, int k
   }
 }
 
+// ---------------------------------------------------------------------------
+// LU panel in registers: rows k0..nrows-1 (R of them), columns k0..k0+w-1
+// (w <= W) of one matrix per workgroup; thread t holds panel rows t + 256 s
+// (s < RP) across all W columns.  Per column: argmax over the thread's
+// rows, wavefront argmax (DPP), merge of the four wavefronts through LDS;
+// the owners of the pivot row and of row j write them to LDS, the two rows
+// trade places and every thread divides its rows by the pivot and updates
+// them in registers.  Two barriers per
+// column.  Rows at or past npiv are updated but never chosen as pivots.
+// ---------------------------------------------------------------------------
+template <typename T, int W, int RP>
+__global__ __launch_bounds__(256) void k_getf2_reg(int batch, int nrows, int k0, int w, int npiv, View<T> A,
+                                                   int* ipiv, int64_t pbs, int* info) {
+  __shared__ T prow[2][W];
+  __shared__ T jrow[2][W];
+  __shared__ T red_v[2][4];
+  __shared__ int red_i[2][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int R = nrows - k0, Rp = npiv - k0;
+  const int rm1 = R - 1, wm1 = w - 1;
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    T x[RP][W];
+#pragma unroll
+    for (int s = 0; s < RP; ++s) {
+      const int rr = tid + 256 * s;
+      const T* rowp = A.at(b, k0 + (rr < rm1 ? rr : rm1), k0);
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const T v = rowp[(int64_t)(c < wm1 ? c : wm1) * A.cs];
+        x[s][c] = (rr < R && c < w) ? v : T(0);
+      }
+    }
+static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
+      constexpr int j = decltype(jc)::value;
+      if (j < w) {
+        const int par = j & 1;
+        T best = T(-1);
+        int bi = INT_MAX;
+#pragma unroll
+        for (int s = 0; s < RP; ++s) {
+          const int rr = tid + 256 * s;
+          const T v = tabs(x[s][j]);
+          const bool ok = (rr >= j) & (rr < Rp) & (v > best);
+          best = ok ? v : best;
+          bi = ok ? rr : bi;
+        }
+        wave_argmax(best, bi);
+        if (lane == 0) {
+          red_v[par][wave] = best;
+          red_i[par][wave] = bi;
+        }
+        __syncthreads();
+        T bv = red_v[par][0];
+        int p = red_i[par][0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) argmax_merge(bv, p, red_v[par][q], red_i[par][q]);
+        if (p < j || p >= Rp) p = j;  // no valid candidate (e.g. a NaN column): keep the diagonal
+        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
+        const int tp = p & 255, sp = p >> 8;
+        if (tid == tp) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) {
+            T v = x[0][c];
+#pragma unroll
+            for (int s = 1; s < RP; ++s) v = bit_blend(v, x[s][c], s == sp);
+            prow[par][c] = v;
+          }
+        }
+        if (tid == j) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) jrow[par][c] = x[0][c];
+        }
+        __syncthreads();
+        if (p != j) {
+          if (tid == tp) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+              const T v = jrow[par][c];
+#pragma unroll
+              for (int s = 0; s < RP; ++s) x[s][c] = bit_blend(x[s][c], v, s == sp);
+            }
+          }
+          if (tid == j) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) x[0][c] = prow[par][c];
+          }
+        }
+        T pr[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) pr[c] = prow[par][c];
+        const T pv = pr[j];
+        const bool nz = pv != T(0);
+        if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+        const T dv = nz ? pv : T(1);
+#pragma unroll
+        for (int s = 0; s < RP; ++s) {
+          const int rr = tid + 256 * s;
+          const bool m = (rr > j) & (rr < R);
+          const T l = x[s][j] / dv;
+          const T lm = m ? l : T(0);
+          x[s][j] = m ? l : x[s][j];
+#pragma unroll
+          for (int c = j + 1; c < W; ++c) x[s][c] -= lm * pr[c];
+        }
+      }
+    });
+#pragma unroll
+    for (int s = 0; s < RP; ++s) {
+      const int rr = tid + 256 * s;
+      if (rr < R) {
+        T* rowp = A.at(b, k0 + rr, k0);
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          if (c < w) rowp[(int64_t)c * A.cs] = x[s][c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// LU panel with one row per thread (R <= NT rows, NT = 256, 512 or 1024
+// threads, 4 to 16 wavefronts so that several share each SIMD and hide each
+// other's dependent-instruction latency).  Per column: wavefront argmax
+// (DPP), every wavefront merges the NT/64 partial results the same way, the
+// owners of the pivot row and of row j swap them through LDS, every thread
+// divides and updates its own row with the pivot row read from LDS.  Two
+// barriers per column; rows at or past npiv are never chosen as pivots.
+template <typename T, int W, int NT>
+__global__ __launch_bounds__(NT) void k_getf2_row(int batch, int nrows, int k0, int w, int npiv, View<T> A,
+                                                  int* ipiv, int64_t pbs, int* info) {
+  constexpr int NW = NT / 64;
+  __shared__ T prow[2][W];
+  __shared__ T jrow[2][W];
+  __shared__ T red_v[2][NW];
+  __shared__ int red_i[2][NW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int R = nrows - k0, Rp = npiv - k0;
+  const int rm1 = R - 1, wm1 = w - 1;
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    T x[W];
+    {
+      const T* rowp = A.at(b, k0 + (tid < rm1 ? tid : rm1), k0);
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const T v = rowp[(int64_t)(c < wm1 ? c : wm1) * A.cs];
+        x[c] = (tid < R && c < w) ? v : T(0);
+      }
+    }
+    static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
+      constexpr int j = decltype(jc)::value;
+      if (j < w) {
+        const int par = j & 1;
+        const bool cand = (tid >= j) & (tid < Rp);
+        T best = cand ? tabs(x[j]) : T(-1);
+        int bi = cand ? tid : INT_MAX;
+        wave_argmax(best, bi);
+        if (lane == 0) {
+          red_v[par][wave] = best;
+          red_i[par][wave] = bi;
+        }
+        __syncthreads();
+        best = red_v[par][lane % NW];
+        bi = red_i[par][lane % NW];
+        wave_argmax(best, bi);
+        int p = bi;
+        if (p < j || p >= Rp) p = j;  // no valid candidate (e.g. a NaN column): keep the diagonal
+        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
+        if (tid == p) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) prow[par][c] = x[c];
+        }
+        if (tid == j) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) jrow[par][c] = x[c];
+        }
+        __syncthreads();
+        if (p != j) {
+          if (tid == p) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) x[c] = jrow[par][c];
+          }
+          if (tid == j) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) x[c] = prow[par][c];
+          }
+        }
+        const T pv = prow[par][j];
+        const bool nz = pv != T(0);
+        if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+        const bool m = (tid > j) & (tid < R);
+        const T l = x[j] / (nz ? pv : T(1));
+        const T lm = m ? l : T(0);
+        x[j] = m ? l : x[j];
+#pragma unroll
+        for (int c = j + 1; c < W; ++c) x[c] -= lm * prow[par][c];
+      }
+    });
+    if (tid < R) {
+      T* rowp = A.at(b, k0 + tid, k0);
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        if (c < w) rowp[(int64_t)c * A.cs] = x[c];
+    }
+    __syncthreads();
+  }
+}
+
+// Row interchanges of one panel (rows k0..k0+w-1, 1-based pivots
+// ipiv[k0..k0+w-1] >= row, w <= W) on every column outside the panel, and
+// U12 = L11^{-1} A12 on the columns right of it; one thread per column
+// (one wavefront per workgroup).  The rows a panel touches are its own w
+// rows plus the distinct pivot rows below it: each gets a slot, the swap
+// sequence is replayed on slot ids (every lane tracks where its slot ends
+// up), and each column then moves its touched entries with independent
+// loads and stores, solving the unit-lower L11 (LDS) on the block rows in
+// registers.
+template <typename T, int W>
+__global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
+                                                  int64_t pbs) {
+  __shared__ T L[W][W + 1];
+  __shared__ int s_piv[W];
+  __shared__ int s_slot[W];
+  __shared__ int s_row[2 * W];
+  __shared__ int s_src[2 * W];
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  const int col = blockIdx.x * 64 + tid;
+  const int cl = col < ncols ? col : ncols - 1;
+  const int wm1 = w - 1;
+  const bool right = col >= k0 + w;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    if (tid < W) {
+      s_piv[tid] = (tid < w) ? ipiv[(int64_t)b * pbs + k0 + tid] - 1 : -1;
+      const int r = tid;
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const T v = *A.at(b, k0 + (r < wm1 ? r : wm1), k0 + (c < wm1 ? c : wm1));
+        L[r][c] = (r < w && c < r) ? v : T(0);
+      }
+    }
+    __syncthreads();
+    const int i = tid;
+    const int p = (i < w) ? s_piv[i] : -1;
+    bool first = (i < w) && (p >= k0 + w);
+    int src_lane = i;
+    for (int q = 0; q < W; ++q) {
+      const bool hit = first & (q < i) & (s_piv[q] == p);
+      src_lane = (hit && src_lane == i) ? q : src_lane;
+    }
+    first = first & (src_lane == i);
+    const unsigned long long mask = __ballot(first);
+    const int rank = __popcll(mask & ((1ull << i) - 1ull));
+    if (first) {
+      s_slot[i] = w + rank;
+      s_row[w + rank] = p;
+    }
+    if (i < w) s_row[i] = k0 + i;
+    if (i == 0) s_cnt = w + __popcll(mask);
+    __syncthreads();
+    if (i < w) {
+      if (p < k0 + w) s_slot[i] = p - k0;
+      else if (!first) s_slot[i] = s_slot[src_lane];
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (i < cnt) {
+      int pos = i;
+      for (int q = 0; q < w; ++q) {
+        const int c = s_slot[q];
+        pos = (pos == q) ? c : ((pos == c) ? q : pos);
+      }
+      s_src[pos] = i;
+    }
+    __syncthreads();
+    T u[W], d[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) u[q] = *A.at(b, s_row[s_src[q < wm1 ? q : wm1]], cl);
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+      const int q = w + t < cnt - 1 ? w + t : cnt - 1;
+      d[t] = *A.at(b, s_row[s_src[q]], cl);
+    }
+    if (right) {
+#pragma unroll
+      for (int a = 0; a < W; ++a)
+#pragma unroll
+        for (int r = a + 1; r < W; ++r) u[r] -= L[r][a] * u[a];
+    }
+    if (col < ncols && (col < k0 || right)) {
+#pragma unroll
+      for (int q = 0; q < W; ++q)
+        if (q < w) *A.at(b, s_row[q], col) = u[q];
+#pragma unroll
+      for (int t = 0; t < W; ++t)
+        if (w + t < cnt) *A.at(b, s_row[w + t], col) = d[t];
+    }
+    __syncthreads();
+  }
+}
+
 static const size_t kPanelLdsBudget = 128 * 1024;
+
+// NT > 0: one row per thread in NT-thread workgroups; NT == 0: RP rows per
+// thread in 256-thread workgroups
+template <typename T, int W, int RP, int NT>
+hipError_t getrf_reg(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs,
+                     int* info, hipStream_t s) {
+  hipError_t e;
+  for (int k0 = 0; k0 < nelim; k0 += W) {
+    const int w = (nelim - k0) < W ? (nelim - k0) : W;
+    if (NT > 0)
+      hipLaunchKernelGGL((k_getf2_row<T, W, (NT > 0 ? NT : 256)>), dim3(grid_batch(batch)), dim3(NT > 0 ? NT : 256),
+                         0, s, batch, nrows, k0, w, npiv, A, ipiv, pbs, info);
+    else
+      hipLaunchKernelGGL((k_getf2_reg<T, W, RP>), dim3(grid_batch(batch)), dim3(256), 0, s, batch, nrows, k0, w,
+                         npiv, A, ipiv, pbs, info);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_swap_trsm<T, W>), dim3(ceil_div(ncols, 64), grid_batch(batch)), dim3(64), 0, s, batch,
+                       ncols, k0, w, A, ipiv, pbs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int mr = nrows - k0 - w, nr = ncols - k0 - w;
+    if (mr > 0 && nr > 0) {
+      e = gemm<T>(batch, mr, nr, w, T(-1), A.sub(k0 + w, k0), A.sub(k0, k0 + w), T(1), A.sub(k0 + w, k0 + w), s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
 
 template <typename T>
 hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs, int* info,
                  hipStream_t s) {
   if (batch <= 0 || nelim <= 0) return hipSuccess;
   if (nelim > npiv || npiv > nrows || nelim > ncols) return hipErrorInvalidValue;
+  // panels held in registers: up to 8 rows per thread (2048 rows)
+  // batched small matrices: 256-thread panels with up to 2 rows per thread;
+  // a single tall matrix: one row per thread in 1024-thread workgroups
+  if (nrows <= 256) return getrf_reg<T, 32, 1, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 512) return getrf_reg<T, 32, 2, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 1024) return getrf_reg<T, 16, 1, 1024>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 2048) return getrf_reg<T, 8, 8, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  // taller panels: LDS-resident panel, generic interchanges
   int nb = 32;
   while (nb > 4 && (size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) nb >>= 1;
   if ((size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) return hipErrorInvalidValue;
@@ -480,8 +847,8 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int mr = nrows - k0 - w, nr = ncols - k0 - w;
     if (nr > 0) {
-      dim3 g3(ceil_div(nr, 128), grid_batch(batch));
-      hipLaunchKernelGGL(k_trsm_diag<T>, g3, dim3(128), 0, s, batch, nr, 0, w, 1, 1, A.sub(k0, k0), A.sub(k0, k0 + w));
+      dim3 g3(ceil_div(nr, 64), grid_batch(batch));
+      hipLaunchKernelGGL(k_trsm_diag<T>, g3, dim3(64), 0, s, batch, nr, 0, w, 1, 1, A.sub(k0, k0), A.sub(k0, k0 + w));
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (mr > 0 && nr > 0) {

@@ -1,0 +1,95 @@
+// Wavefront (64-lane) primitives for gfx950: lane broadcasts and an argmax
+// all-reduce built on DPP row permutations plus four readlanes, so a pivot
+// search costs a few VALU operations instead of LDS round trips.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <type_traits>
+
+namespace slab {
+
+// Broadcast lane j's value of v to the wavefront (j uniform).
+__device__ inline int rdlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ inline float rdlane(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double rdlane(double v, int j) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v from the lane selected by the DPP control CTRL (all lanes valid sources)
+template <int CTRL>
+__device__ inline int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ inline float dpp(float v) {
+  return __int_as_float(dpp<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ inline double dpp(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp<CTRL>((int)(unsigned)u);
+  const unsigned hi = (unsigned)dpp<CTRL>((int)(unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// (v, i) <- the larger value, ties to the smaller index (LAPACK i?amax order)
+template <typename T>
+__device__ inline void argmax_merge(T& v, int& i, T ov, int oi) {
+  const bool take = (ov > v) | ((ov == v) & (oi < i));
+  v = take ? ov : v;
+  i = take ? oi : i;
+}
+
+// All-reduce argmax over the wavefront: lanes pair up by quad permutations
+// (xor 1, xor 2), then half-row and row mirrors (which pair the two quads of
+// each 8 and the two 8s of each 16), then the four 16-lane rows are merged
+// from their first lanes.  Every lane returns the same (v, i).
+template <typename T>
+__device__ inline void wave_argmax(T& v, int& i) {
+  argmax_merge(v, i, dpp<0xB1>(v), dpp<0xB1>(i));    // quad_perm [1,0,3,2]
+  argmax_merge(v, i, dpp<0x4E>(v), dpp<0x4E>(i));    // quad_perm [2,3,0,1]
+  argmax_merge(v, i, dpp<0x141>(v), dpp<0x141>(i));  // row_half_mirror
+  argmax_merge(v, i, dpp<0x140>(v), dpp<0x140>(i));  // row_mirror
+  T v0 = rdlane(v, 0);
+  int i0 = rdlane(i, 0);
+  argmax_merge(v0, i0, rdlane(v, 16), rdlane(i, 16));
+  argmax_merge(v0, i0, rdlane(v, 32), rdlane(i, 32));
+  argmax_merge(v0, i0, rdlane(v, 48), rdlane(i, 48));
+  v = v0;
+  i = i0;
+}
+
+// Bitwise blend: (a & ~m) | (b & m) on the bit patterns, m all ones or all
+// zeros.  Picking one of several register-resident values this way keeps
+// every index compile-time (a select chain would be folded into a load
+// through a selected address, which forces the array into scratch).
+__device__ inline float bit_blend(float a, float b, bool take) {
+  const unsigned m = take ? 0xFFFFFFFFu : 0u;
+  return __uint_as_float((__float_as_uint(a) & ~m) | (__float_as_uint(b) & m));
+}
+__device__ inline double bit_blend(double a, double b, bool take) {
+  const unsigned long long m = take ? ~0ull : 0ull;
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  return __longlong_as_double((long long)((ua & ~m) | (ub & m)));
+}
+
+// f(std::integral_constant<int, J>) for J = B..E-1: loop bodies whose index
+// must be a compile-time constant (register-resident arrays), independent of
+// the unroller's size limits.
+template <int B, int E, typename F>
+__device__ __attribute__((always_inline)) inline void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+}  // namespace slab
