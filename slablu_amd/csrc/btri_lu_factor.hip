@@ -9,7 +9,7 @@ namespace {
 
 // One workgroup per chain for the whole factorization.  Per block: stage
 // D_i, subtract Lfac_{i-1} Ufac_{i-1} (products from global in KC-wide LDS
-// chunks), LU with partial pivoting in LDS, then Ufac_i (rows gathered
+// chunks), blocked LU with partial pivoting in LDS, then Ufac_i (rows gathered
 // through the composed pivot permutation) and Lfac_i (transposed) in column
 // chunks of XC by triangular solves against the LDS-resident factors.
 template <typename T, int RT>
@@ -18,15 +18,15 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, B
   constexpr int BP = 16 * RT;
   extern __shared__ unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);  // BP x (BP+1), padded diagonal block
-  T* X = S + BP * (BP + 1);                // bs x (XC+1) right-hand-side chunk
-  T* TA = X + bs * (XC + 1);               // BP x (KC+1)
+  const int ldx = lu_factor_ldx(XC);
+  T* X = S + BP * (BP + 1);                // bs x ldx right-hand-side chunk
+  T* TA = X + bs * ldx;                    // BP x (KC+1)
   T* TB = TA + BP * (KC + 1);              // KC x (BP+1)
   int* piv = reinterpret_cast<int*>(TB + KC * (BP + 1));
   int* Q = piv + bs;
-  __shared__ T red_v[4];
-  __shared__ int red_i[4];
+  __shared__ __align__(16) int wsp[6 * 16 + 2 * 16 * (int)(sizeof(T) / sizeof(int))];
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const int ld = BP + 1, ldx = XC + 1;
+  const int ld = BP + 1;
   const float rbs = 1.0f / (float)bs;
   const int nn = bs * bs;
   auto Lat = [&](int r, int k) -> T { return S[r * ld + k]; };
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int nb, int m, int bs, B
           for (int y = 0; y < RT; ++y) S[(ty + 16 * x) * ld + tx + 16 * y] -= acc[x][y];
         __syncthreads();
       }
-      lu_lds<T, RT>(S, bs, ld, piv, red_v, red_i, info, b);
+      lu_lds_blocked<T, RT>(S, bs, ld, piv, wsp, info, b);
       for (int e = tid; e < nn; e += 256) {
         const int r = idiv(e, rbs);
         Di[e] = S[r * ld + e - r * bs];

@@ -197,16 +197,16 @@ __device__ void tri_solve_wave(At at, int n, T* X, int ldx, int nc) {
   }
 }
 
-// Triangular solve M X = X in LDS for many columns: each 16-row diagonal
-// block is solved by one thread per column in registers, then the rows
-// beyond it are updated by one thread per (column, row group) holding the
-// block's 16 solved values.  Reads past the last row are clamped to it;
-// the values they produce only reach rows that are never stored (lower) or
-// are multiplied by zeros (upper).
-template <typename T, bool UPPER, bool UNIT, typename At>
-__device__ void tri_solve_cols(At at, int n, T* X, int ldx, int nc) {
+// Triangular solve M X = X in LDS for 8 < nc <= 16 CQ columns with X row
+// stride ldx >= 16 CQ (columns nc..16CQ-1 of X are scratch): each 16-row
+// diagonal block is solved by one thread per column in registers, then
+// thread (ty, tx) of a 16 x 16 arrangement updates rows ty + 16i beyond the
+// block on columns tx + 16q (q < CQ) holding the block's solved values of
+// its CQ columns in registers, so every read of M feeds CQ FMAs.
+template <typename T, bool UPPER, bool UNIT, int CQ, typename At>
+__device__ void tri_solve_tiled(At at, int n, T* X, int ldx, int nc) {
   constexpr int DB = 16;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int nblk = (n + DB - 1) / DB;
   const int nm1 = n - 1;
   for (int bi = 0; bi < nblk; ++bi) {
@@ -248,113 +248,301 @@ __device__ void tri_solve_cols(At at, int n, T* X, int ldx, int nc) {
     const int r0 = UPPER ? 0 : d0 + tb;
     const int r1 = UPPER ? d0 : n;
     if (r1 > r0) {
-      for (int cc = 0; cc < nc; cc += 256) {
-        const int nch = (nc - cc) < 256 ? (nc - cc) : 256;
-        const int G = 256 / nch;
-        const int c = cc + tid % nch, g = tid / nch;
-        if (g < G) {
-          T xc[DB];
+      T xc[CQ][DB];
 #pragma unroll
-          for (int j = 0; j < DB; ++j) {
-            const T v = X[(d0 + (j < tb ? j : tb - 1)) * ldx + c];
-            xc[j] = (j < tb) ? v : T(0);
-          }
-          for (int r = r0 + g; r < r1; r += G) {
-            T acc = T(0);
+      for (int q = 0; q < CQ; ++q)
 #pragma unroll
-            for (int j = 0; j < DB; ++j) acc += at(r, (d0 + j) < nm1 ? (d0 + j) : nm1) * xc[j];
-            X[r * ldx + c] -= acc;
-          }
+        for (int j = 0; j < DB; ++j) {
+          const T v = X[(d0 + (j < tb ? j : tb - 1)) * ldx + tx + 16 * q];
+          xc[q][j] = (j < tb) ? v : T(0);
         }
+      for (int r = r0 + ty; r < r1; r += 16) {
+        T acc[CQ];
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) acc[q] = T(0);
+#pragma unroll
+        for (int j = 0; j < DB; ++j) {
+          const T a = at(r, (d0 + j) < nm1 ? (d0 + j) : nm1);
+#pragma unroll
+          for (int q = 0; q < CQ; ++q) acc[q] += a * xc[q][j];
+        }
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) X[r * ldx + tx + 16 * q] -= acc[q];
       }
     }
     __syncthreads();
   }
 }
 
+// X (n x nc, row stride ldx) <- M^{-1} X for a triangular M read through
+// `at`.  For nc > 8 the columns go in chunks of up to 64 (FP64) or 128
+// (FP32) with 16 CQ columns per tile row, so ldx must be at least nc
+// rounded up to 16; the padding columns are scratch.
 template <typename T, bool UPPER, bool UNIT, typename At>
 __device__ inline void tri_solve(At at, int n, T* X, int ldx, int nc) {
-  if (nc == 1)
+  if (nc == 1) {
     tri_solve_wave<T, UPPER, UNIT, 1>(at, n, X, ldx, nc);
-  else if (nc == 2)
+    return;
+  }
+  if (nc == 2) {
     tri_solve_wave<T, UPPER, UNIT, 2>(at, n, X, ldx, nc);
-  else if (nc <= 4)
+    return;
+  }
+  if (nc <= 4) {
     tri_solve_wave<T, UPPER, UNIT, 4>(at, n, X, ldx, nc);
-  else if (nc <= 8)
+    return;
+  }
+  if (nc <= 8) {
     tri_solve_wave<T, UPPER, UNIT, 8>(at, n, X, ldx, nc);
-  else
-    tri_solve_cols<T, UPPER, UNIT>(at, n, X, ldx, nc);
+    return;
+  }
+  constexpr int CQM = sizeof(T) == 8 ? 4 : 8;
+  for (int c0 = 0; c0 < nc; c0 += 16 * CQM) {
+    const int w = (nc - c0) < 16 * CQM ? (nc - c0) : 16 * CQM;
+    switch ((w + 15) >> 4) {
+      case 1: tri_solve_tiled<T, UPPER, UNIT, 1>(at, n, X + c0, ldx, w); break;
+      case 2: tri_solve_tiled<T, UPPER, UNIT, 2>(at, n, X + c0, ldx, w); break;
+      case 3: tri_solve_tiled<T, UPPER, UNIT, 3>(at, n, X + c0, ldx, w); break;
+      case 4: tri_solve_tiled<T, UPPER, UNIT, 4>(at, n, X + c0, ldx, w); break;
+      default:
+        if constexpr (CQM > 4) {
+          switch ((w + 15) >> 4) {
+            case 5: tri_solve_tiled<T, UPPER, UNIT, 5>(at, n, X + c0, ldx, w); break;
+            case 6: tri_solve_tiled<T, UPPER, UNIT, 6>(at, n, X + c0, ldx, w); break;
+            case 7: tri_solve_tiled<T, UPPER, UNIT, 7>(at, n, X + c0, ldx, w); break;
+            default: tri_solve_tiled<T, UPPER, UNIT, 8>(at, n, X + c0, ldx, w); break;
+          }
+        }
+        break;
+    }
+  }
 }
 
-// In-place LU with partial pivoting of the n x n matrix S in LDS (row stride
-// ld >= 16RT + 1, n <= 16RT): thread (ty, tx) of a 16 x 16 arrangement owns
-// rows ty + 16a and columns tx + 16q of the padded 16RT x 16RT tile and
-// updates it with no bounds tests (padding rows/columns are masked out
-// through zero multipliers and hold garbage on return).  Per column: pivot
-// search (wavefront reductions, every thread then picks the winner of the
-// four wavefronts), row interchange, rank-1 update; the multipliers of
-// column k are written back during the pivot search of column k + 1, so
-// each column costs three barriers.  piv[k] is 0-based.
+// Slots of the rows one LU panel touches (its rows k0..k0+nb-1 and the
+// distinct pivot rows below it; piv holds 0-based absolute pivots) and, per
+// slot, the slot whose entry lands there after the interchanges.  Run by one
+// wavefront (lanes = panel rows); writes s_row, s_src (2nb each), *s_cnt.
+__device__ inline void panel_slots(const int* piv, int k0, int nb, int* s_slot, int* s_row, int* s_src,
+                                   int* s_cnt) {
+  const int i = threadIdx.x & 63;
+  const int p = (i < nb) ? piv[k0 + i] : -1;
+  bool first = (i < nb) && (p >= k0 + nb);
+  int src = i;
+  for (int q = 0; q < nb; ++q) {
+    const bool hit = first & (q < i) & (piv[k0 + q] == p);
+    src = (hit && src == i) ? q : src;
+  }
+  first = first & (src == i);
+  const unsigned long long mask = __ballot(first);
+  const int rank = __popcll(mask & ((1ull << i) - 1ull));
+  if (first) {
+    s_slot[i] = nb + rank;
+    s_row[nb + rank] = p;
+  }
+  if (i < nb) s_row[i] = k0 + i;
+  const int cnt = nb + __popcll(mask);
+  if (i == 0) *s_cnt = cnt;
+  __builtin_amdgcn_wave_barrier();
+  if (i < nb) {
+    if (p < k0 + nb) s_slot[i] = p - k0;
+    else if (!first) s_slot[i] = s_slot[src];
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (i < cnt) {
+    int pos = i;
+    for (int q = 0; q < nb; ++q) {
+      const int c = s_slot[q];
+      pos = (pos == q) ? c : ((pos == c) ? q : pos);
+    }
+    s_src[pos] = i;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Blocked in-place LU with partial pivoting of the n x n matrix S in LDS
+// (row stride ld >= 16RT + 1, padded to 16RT rows and columns, n <= 16RT).
+// Each 16-column panel is factored by wavefront 0 alone with the panel in
+// registers (lane l holds panel rows l + 64s): DPP argmax pivot search,
+// pivot row and row exchange through LDS, division by the pivot, rank-1
+// update in registers; no workgroup barrier inside the panel.  Then one
+// thread per column outside the panel moves the entries the panel's
+// interchanges touch (slots from panel_slots) and, right of the panel,
+// solves with the unit-lower L11; then every thread updates its register
+// tile of the trailing matrix (multipliers of finished rows and entries
+// of padding rows/columns enter as zeros).  Four barriers per panel.
+// piv[k] is 0-based.  wsp: LDS workspace of 6 * 16 ints + 32 T.
 template <typename T, int RT>
-__device__ void lu_lds(T* S, int n, int ld, int* piv, T* red_v, int* red_i, int* info, int64_t b) {
+__device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* info, int64_t b) {
+  constexpr int NB = 16;
+  constexpr int RPW = (16 * RT + 63) / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tx = tid & 15, ty = tid >> 4;
-  const int nm1 = n - 1;
-  T colk[RT];
-  for (int k = 0; k < n; ++k) {
-    if (k > 0 && tx == ((k - 1) & 15)) {
+  int* s_slot = wsp;                 // NB
+  int* s_row = wsp + NB;             // 2 NB
+  int* s_src = wsp + 3 * NB;         // 2 NB
+  int* s_cnt = wsp + 5 * NB;         // 1
+  T* xch = reinterpret_cast<T*>(wsp + 6 * NB);  // 2 NB
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = (n - k0) < NB ? (n - k0) : NB;
+    const int R = n - k0;
+    if (wave == 0) {
+      T x[RPW][NB];
 #pragma unroll
-      for (int a = 0; a < RT; ++a) {
-        const int r = ty + 16 * a;
-        if (r > k - 1 && r < n) S[r * ld + k - 1] = colk[a];
+      for (int s = 0; s < RPW; ++s) {
+        const int rr = lane + 64 * s;
+        const int rc = rr < R - 1 ? rr : R - 1;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          const T v = S[(k0 + rc) * ld + k0 + c];
+          x[s][c] = (rr < R && c < nb) ? v : T(0);
+        }
+      }
+      static_for<0, NB>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if (j < nb) {
+          T best = T(-1);
+          int bi = INT_MAX;
+#pragma unroll
+          for (int s = 0; s < RPW; ++s) {
+            const int rr = lane + 64 * s;
+            const T v = absv(x[s][j]);
+            const bool ok = (rr >= j) & (rr < R) & (v > best);
+            best = ok ? v : best;
+            bi = ok ? rr : bi;
+          }
+          wave_argmax(best, bi);
+          const int p = (bi < j || bi >= R) ? j : bi;
+          if (lane == 0) piv[k0 + j] = k0 + p;
+          const int lp = p & 63, sp = p >> 6;
+          if (lane == lp) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+              T v = x[0][c];
+#pragma unroll
+              for (int s = 1; s < RPW; ++s) v = bit_blend(v, x[s][c], s == sp);
+              xch[c] = v;
+            }
+          }
+          if (lane == j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) xch[NB + c] = x[0][c];
+          }
+          __builtin_amdgcn_wave_barrier();
+          T pr[NB];
+#pragma unroll
+          for (int c = 0; c < NB; ++c) pr[c] = xch[c];
+          if (p != j) {
+            if (lane == lp) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) {
+                const T v = xch[NB + c];
+#pragma unroll
+                for (int s = 0; s < RPW; ++s) x[s][c] = bit_blend(x[s][c], v, s == sp);
+              }
+            }
+            if (lane == j) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) x[0][c] = pr[c];
+            }
+          }
+          const T pv = pr[j];
+          const bool nz = pv != T(0);
+          if (!nz && lane == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+          const T dv = nz ? pv : T(1);
+#pragma unroll
+          for (int s = 0; s < RPW; ++s) {
+            const int rr = lane + 64 * s;
+            const bool m = (rr > j) & (rr < R);
+            const T l = x[s][j] / dv;
+            const T lm = m ? l : T(0);
+            x[s][j] = m ? l : x[s][j];
+#pragma unroll
+            for (int c = j + 1; c < NB; ++c) x[s][c] -= lm * pr[c];
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      });
+#pragma unroll
+      for (int s = 0; s < RPW; ++s) {
+        const int rr = lane + 64 * s;
+        if (rr < R) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < nb) S[(k0 + rr) * ld + k0 + c] = x[s][c];
+        }
+      }
+      panel_slots(piv, k0, nb, s_slot, s_row, s_src, s_cnt);
+    }
+    __syncthreads();
+    // interchanges outside the panel and U12 = L11^{-1} A12
+    {
+      const int c = tid;
+      const int cnt = *s_cnt;
+      const int cc = c < n - 1 ? c : n - 1;
+      const bool right = c >= k0 + nb;
+      T u[NB], d[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) u[q] = S[s_row[s_src[q < nb - 1 ? q : nb - 1]] * ld + cc];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const int q = nb + t < cnt - 1 ? nb + t : cnt - 1;
+        d[t] = S[s_row[s_src[q]] * ld + cc];
+      }
+      if (right) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const T ua = (a < nb) ? u[a] : T(0);
+#pragma unroll
+          for (int r = a + 1; r < NB; ++r) {
+            const T l = S[(k0 + (r < nb ? r : nb - 1)) * ld + k0 + a];
+            u[r] -= ((r < nb) ? l : T(0)) * ua;
+          }
+        }
+      }
+      if (c < n && (c < k0 || right)) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          if (q < nb) S[s_row[q] * ld + c] = u[q];
+#pragma unroll
+        for (int t = 0; t < NB; ++t)
+          if (nb + t < cnt) S[s_row[nb + t] * ld + c] = d[t];
       }
     }
-    T best = T(-1);
-    int bi = INT_MAX;
-    for (int r0 = k; r0 < n; r0 += 256) {
-      const int r = r0 + tid;
-      const T v = absv(S[(r < nm1 ? r : nm1) * ld + k]);
-      const bool ok = r < n && v > best;
-      best = ok ? v : best;
-      bi = ok ? r : bi;
-    }
-    wave_argmax(best, bi);
-    if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
     __syncthreads();
-    T bv = red_v[0];
-    int p = red_i[0];
+    // trailing update: rows and columns k0+nb .. n-1
+    {
+      const int kk = k0 + nb;
+      if (kk < n) {
+        T acc[RT][RT];
 #pragma unroll
-    for (int q = 1; q < 4; ++q) argmax_merge(bv, p, red_v[q], red_i[q]);
-    if (p < k || p >= n) p = k;
-    if (tid == 0) piv[k] = p;
-    if (p != k) {
-      for (int c = tid; c < n; c += 256) {
-        const T t = S[k * ld + c];
-        S[k * ld + c] = S[p * ld + c];
-        S[p * ld + c] = t;
+        for (int a = 0; a < RT; ++a)
+#pragma unroll
+          for (int q = 0; q < RT; ++q) acc[a][q] = T(0);
+        for (int j = 0; j < nb; ++j) {
+          T av[RT], bv[RT];
+#pragma unroll
+          for (int a = 0; a < RT; ++a) {
+            const int r = ty + 16 * a;
+            const T v = S[r * ld + k0 + j];
+            av[a] = (r >= kk && r < n) ? v : T(0);
+          }
+#pragma unroll
+          for (int q = 0; q < RT; ++q) {
+            const int c = tx + 16 * q;
+            const T v = S[(k0 + j) * ld + c];
+            bv[q] = (c >= kk && c < n) ? v : T(0);
+          }
+#pragma unroll
+          for (int a = 0; a < RT; ++a)
+#pragma unroll
+            for (int q = 0; q < RT; ++q) acc[a][q] += av[a] * bv[q];
+        }
+#pragma unroll
+        for (int a = 0; a < RT; ++a)
+#pragma unroll
+          for (int q = 0; q < RT; ++q) S[(ty + 16 * a) * ld + tx + 16 * q] -= acc[a][q];
       }
     }
-    __syncthreads();
-    const T pv = S[k * ld + k];
-    const bool nz = pv != T(0);
-    if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k + 1;
-    T rowk[RT];
-#pragma unroll
-    for (int q = 0; q < RT; ++q) {
-      const int c = tx + 16 * q;
-      const T v = S[k * ld + c];
-      rowk[q] = (c > k && c < n) ? v : T(0);
-    }
-#pragma unroll
-    for (int a = 0; a < RT; ++a) {
-      const int r = ty + 16 * a;
-      const T v = S[r * ld + k];
-      colk[a] = (r > k && r < n) ? (nz ? v / pv : v) : T(0);
-    }
-#pragma unroll
-    for (int a = 0; a < RT; ++a)
-#pragma unroll
-      for (int q = 0; q < RT; ++q) S[(ty + 16 * a) * ld + tx + 16 * q] -= colk[a] * rowk[q];
     __syncthreads();
   }
 }
@@ -443,12 +631,16 @@ constexpr size_t kLdsMax = 160 * 1024;
 
 // 16 x 16 thread tiles per dimension of a bs x bs block (the RT of the LU
 // and Gauss-Jordan kernels, which pad blocks to 16 RT).
-inline int tile_rt(int bs) { return (bs + 15) / 16; }
+__host__ __device__ inline int tile_rt(int bs) { return (bs + 15) / 16; }
+
+// row stride of the right-hand-side chunk (xc columns rounded up to 16 for
+// the tiled triangular solves, plus one)
+__host__ __device__ inline int lu_factor_ldx(int xc) { return ((xc + 15) & ~15) + 1; }
 
 template <typename T>
 size_t lu_factor_lds(int bs, int xc) {
   const size_t bp = 16 * (size_t)tile_rt(bs);
-  return sizeof(T) * (bp * (bp + 1) + (size_t)bs * (xc + 1) + bp * (KC + 1) + (size_t)KC * (bp + 1)) +
+  return sizeof(T) * (bp * (bp + 1) + (size_t)bs * lu_factor_ldx(xc) + bp * (KC + 1) + (size_t)KC * (bp + 1)) +
          2 * sizeof(int) * bs;
 }
 
