@@ -2,12 +2,25 @@
 src_solver/block_tridiag_solver_structured.py of the reference).
 
 The reference keeps the interface blocks HBS-compressed and densifies each
-one before its LU (block_tridiag_solver_structured.py:34-76).  Here the
-blocks are formed exactly and densely on the GPU, and the Schur recursion
-S_0 = D_0, S_j = D_j - sub_{j-1} S_{j-1}^{-1} sup_{j-1} is run as one
-(b = 1) block-tridiagonal chain in block-Thomas form with S_j^{-1} and
-G_j = S_j^{-1} sup_j kept (slab_btri_inv_factor, FP64), so that the solve
-is a sequence of matrix products.
+one before its LU (block_tridiag_solver_structured.py:34-76), running the
+Schur recursion S_0 = D_0, S_j = D_j - sub_{j-1} S_{j-1}^{-1} sup_{j-1} from
+the first block to the last.  Here the blocks are formed exactly and densely
+on the GPU and eliminated from both ends at once (a twisted block
+factorization): the top chain runs the same recursion over blocks 0..k-1,
+the bottom chain runs its mirror image over blocks m-1..k+1,
+
+    T_{m-1} = D_{m-1},  T_i = D_i - sup_i T_{i+1}^{-1} sub_i,
+
+and the two meet at the middle block k = m // 2,
+
+    M = D_k - sub_{k-1} S_{k-1}^{-1} sup_{k-1} - sup_k T_{k+1}^{-1} sub_k.
+
+Each step factors one block of each chain as a batch of two, so the number
+of dependent dense LU factorizations drops from m to m // 2 + 1 for the same
+block products.  Each block is kept in inverse form (S^{-1} and
+G = S^{-1} * coupling towards the middle, FP64), so a solve is a sequence of
+matrix products: forward sweeps down both chains, the middle block, then
+back-substitution out to both ends.
 """
 import torch
 
@@ -22,6 +35,17 @@ def _dense(x, device):
     return torch.as_tensor(x).to(device)
 
 
+def _inverse(S):
+    """S (b, n, n) -> S^{-1} (b, n, n) through getrf + getrs on the identity."""
+    b, n, _ = S.shape
+    ipiv = torch.empty(b, n, dtype=torch.int32, device=S.device)
+    info = torch.zeros(b, dtype=torch.int32, device=S.device)
+    hip.getrf(S, ipiv, info=info)
+    X = torch.eye(n, dtype=S.dtype, device=S.device).expand(b, n, n).contiguous()
+    hip.getrs(S, ipiv, X)
+    return X, info
+
+
 class HBS_BlockTridiagonal:
     def __init__(self, b, n, cyclic=False):
         assert b < n
@@ -33,29 +57,104 @@ class HBS_BlockTridiagonal:
         """diag = diag0 + diag1 (b, n, n); sub, sup (b-1, n, n); dense
         tensors or objects with todense()."""
         device = torch.device("cuda") if device is None else torch.device(device)
-        b, n = self.b, self.n
+        m, n = self.b, self.n
         diag = _dense(hbs_diag0, device).to(dtype)
         if hbs_diag1 is not None:
             diag = diag + _dense(hbs_diag1, device).to(dtype)
-        self.diag = diag.reshape(1, b, n, n).contiguous()
-        if b > 1:
-            self.sub = _dense(hbs_sub, device).to(dtype).reshape(1, b - 1, n, n).contiguous()
-            self.sup = _dense(hbs_sup, device).to(dtype).reshape(1, b - 1, n, n).contiguous()
+        diag = diag.reshape(m, n, n)
+        if m > 1:
+            sub = _dense(hbs_sub, device).to(dtype).reshape(m - 1, n, n)
+            sup = _dense(hbs_sup, device).to(dtype).reshape(m - 1, n, n)
+        k = m // 2
+        L = k                                    # chain length (bottom padded)
+        self.k, self.L = k, L
+        self.info = torch.zeros(2, dtype=torch.int32, device=device)
+        if L > 0:
+            pad = L - (m - 1 - k)                # identity blocks in front of the bottom chain
+            Dc = torch.zeros(2, L, n, n, dtype=dtype, device=device)
+            Cl = torch.zeros(2, L, n, n, dtype=dtype, device=device)   # coupling to the previous chain block
+            Cr = torch.zeros(2, L, n, n, dtype=dtype, device=device)   # coupling to the next block (last: middle)
+            for t in range(L):
+                Dc[0, t] = diag[t]
+                if t > 0:
+                    Cl[0, t] = sub[t - 1]
+                Cr[0, t] = sup[t]
+                if t < pad:
+                    Dc[1, t] = torch.eye(n, dtype=dtype, device=device)
+                else:
+                    i = m - 1 - (t - pad)
+                    Dc[1, t] = diag[i]
+                    if t > pad:
+                        Cl[1, t] = sup[i]
+                    Cr[1, t] = sub[i - 1]
+            Sinv = torch.empty_like(Dc)
+            G = torch.empty_like(Dc)
+            for t in range(L):
+                S = Dc[:, t].clone()
+                if t > 0:
+                    hip.gemm(Cl[:, t], G[:, t - 1], S, alpha=-1.0, beta=1.0)
+                Sinv[:, t], info = _inverse(S)
+                self.info = torch.maximum(self.info, info)
+                hip.gemm(Sinv[:, t], Cr[:, t], G[:, t])
+            M = diag[k].clone().unsqueeze(0)
+            hip.gemm(sub[k - 1], G[0, L - 1].unsqueeze(0), M, alpha=-1.0, beta=1.0)
+            self.has_bot = m - 1 - k > 0
+            if self.has_bot:
+                hip.gemm(sup[k], G[1, L - 1].unsqueeze(0), M, alpha=-1.0, beta=1.0)
+            self.Sinv, self.G, self.Cl = Sinv, G, Cl
+            self.sub_mid = sub[k - 1].contiguous()
+            self.sup_mid = sup[k].contiguous() if self.has_bot else None
+            self.pad = pad
+            del Dc, Cr
         else:
-            self.sub = torch.zeros(1, 1, n, n, device=device, dtype=dtype)
-            self.sup = torch.zeros(1, 1, n, n, device=device, dtype=dtype)
-        self.info = torch.zeros(1, dtype=torch.int32, device=device)
-        hip.btri_inv_factor(self.diag, self.sub, self.sup, info=self.info)
-        self.nbytes = (self.diag.numel() + self.sub.numel() + self.sup.numel()) * self.diag.element_size()
+            M = diag[k].clone().unsqueeze(0)
+        self.Minv, info = _inverse(M)
+        self.info[0] = torch.maximum(self.info[0], info[0])
+        self.nbytes = sum(t.numel() * t.element_size()
+                          for t in ([self.Minv] + ([self.Sinv, self.G, self.Cl] if L > 0 else [])))
+        self.dtype, self.device = dtype, device
         self.build_mode = "lowmem"
 
     build = build_lowmem
 
     def solve(self, f):
         """f (b*n, nrhs) -> solution (b*n, nrhs); f is not modified."""
+        m, n, k, L = self.b, self.n, self.k, self.L
         nrhs = f.shape[-1]
-        x = f.to(self.diag.device, self.diag.dtype).reshape(1, self.b * self.n, nrhs).contiguous().clone()
-        hip.btri_inv_solve(self.diag, self.sub, self.sup, x)
-        return x.reshape(self.b * self.n, nrhs)
+        fb = f.to(self.device, self.dtype).reshape(m, n, nrhs)
+        x = torch.empty(m, n, nrhs, dtype=self.dtype, device=self.device)
+        rhs_mid = fb[k].clone().unsqueeze(0)
+        if L > 0:
+            pad = self.pad
+            Y = torch.zeros(2, L, n, nrhs, dtype=self.dtype, device=self.device)
+            top = list(range(L))
+            bot = [m - 1 - (t - pad) if t >= pad else None for t in range(L)]
+            W = torch.empty(2, n, nrhs, dtype=self.dtype, device=self.device)
+            for t in range(L):
+                W[0] = fb[top[t]]
+                if bot[t] is None:
+                    W[1] = 0
+                else:
+                    W[1] = fb[bot[t]]
+                if t > 0:
+                    hip.gemm(self.Cl[:, t], Y[:, t - 1], W, alpha=-1.0, beta=1.0)
+                hip.gemm(self.Sinv[:, t], W, Y[:, t])
+            hip.gemm(self.sub_mid, Y[0, L - 1].unsqueeze(0), rhs_mid, alpha=-1.0, beta=1.0)
+            if self.has_bot:
+                hip.gemm(self.sup_mid, Y[1, L - 1].unsqueeze(0), rhs_mid, alpha=-1.0, beta=1.0)
+        xk = torch.empty(1, n, nrhs, dtype=self.dtype, device=self.device)
+        hip.gemm(self.Minv, rhs_mid, xk)
+        x[k] = xk[0]
+        if L > 0:
+            X = torch.empty(2, L, n, nrhs, dtype=self.dtype, device=self.device)
+            nxt = xk.expand(2, n, nrhs)
+            for t in range(L - 1, -1, -1):
+                X[:, t] = Y[:, t]
+                hip.gemm(self.G[:, t], nxt, X[:, t], alpha=-1.0, beta=1.0)
+                nxt = X[:, t]
+            x[:L] = X[0]
+            for t in range(pad, L):
+                x[m - 1 - (t - pad)] = X[1, t]
+        return x.reshape(m * n, nrhs)
 
     solve_lowmem = solve

@@ -304,14 +304,17 @@ def test_block_tridiagonal_api(form):
         assert _rel(xt, torch.linalg.solve(M.transpose(1, 2), f0)) < 1e-11
 
 
-def test_hbs_block_tridiagonal_api():
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 9])
+def test_hbs_block_tridiagonal_api(nb):
+    """Twisted (two-ended) block elimination against a dense solve, for odd,
+    even and degenerate block counts."""
     from slablu_amd.src_solver.block_tridiag_solver_structured import HBS_BlockTridiagonal
-    g = torch.Generator().manual_seed(9)
-    nb, n = 4, 300
+    g = torch.Generator().manual_seed(9 + nb)
+    n = 300
     d0 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) + 5 * torch.eye(n, dtype=torch.float64)
     d1 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) * 0.1
-    sub = torch.randn(nb - 1, n, n, generator=g, dtype=torch.float64) * 0.2
-    sup = torch.randn(nb - 1, n, n, generator=g, dtype=torch.float64) * 0.2
+    sub = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.2
+    sup = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.2
     H = HBS_BlockTridiagonal(nb, n)
     H.build_lowmem(d0, d1, sub, sup, device="cuda")
     f = torch.randn(nb * n, 2, generator=g, dtype=torch.float64)
