@@ -93,6 +93,35 @@ int slab_btri_solve(int dtype, int b, int m, int bs,
                     const int* ipiv, int64_t p_bstride, int64_t p_istride,
                     void* f, int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream);
 
+/* Twisted variant of slab_btri_factor (same arguments and storage): the
+ * blocks 0..k-1 (k = m/2) are eliminated top-down and m-1..k+1 bottom-up,
+ * both chains of every system concurrently, then the middle block k.  The
+ * factors are only meaningful to slab_btri_solve_twisted.  The same
+ * Gaussian elimination as build_sweep
+ * (src_solver/block_tridiag_solver_dense.py:152-177) in a different block
+ * order, with half the dependent steps.  Requires
+ * slab_btri_fused(dtype, bs) == 1; returns -1 otherwise. */
+int slab_btri_factor_twisted(int dtype, int b, int m, int bs,
+                             void* diag, int64_t d_bstride, int64_t d_istride,
+                             void* sub, int64_t l_bstride, int64_t l_istride,
+                             void* sup, int64_t u_bstride, int64_t u_istride,
+                             int* ipiv, int64_t p_bstride, int64_t p_istride, int* info, void* stream);
+
+/* Sweeps with the factors of slab_btri_factor_twisted, in place on f (as
+ * slab_btri_solve, non-transposed).  Replaces lusolve_sweep
+ * (src_solver/block_tridiag_solver_dense.py:179-202) for twisted factors. */
+int slab_btri_solve_twisted(int dtype, int b, int m, int bs,
+                            const void* diag, int64_t d_bstride, int64_t d_istride,
+                            const void* sub, int64_t l_bstride, int64_t l_istride,
+                            const void* sup, int64_t u_bstride, int64_t u_istride,
+                            const int* ipiv, int64_t p_bstride, int64_t p_istride,
+                            void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* stream);
+
+/* 1 when blocks of size bs run in the fused one-workgroup-per-chain LU-form
+ * kernels (the diagonal block, a right-hand-side chunk and staging fit in
+ * LDS), 0 otherwise, -1 for a bad dtype. */
+int slab_btri_fused(int dtype, int bs);
+
 /* Block-tridiagonal factorization in block-Thomas form with explicit
  * diagonal inverses (same chains and storage as slab_btri_factor):
  *   S_0 = D_0, S_i = D_i - sub_{i-1} G_{i-1}, G_i = S_i^{-1} sup_i;

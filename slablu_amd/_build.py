@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libslabhip.so")
-SOURCES = ["linalg.hip", "leaf.hip", "btri_inv.hip", "btri_lu_factor.hip", "btri_lu_solve.hip", "getrs_small.hip", "api.hip"]
+SOURCES = ["linalg.hip", "leaf.hip", "btri_inv.hip", "btri_lu_factor_f32.hip", "btri_lu_factor_f64.hip", "btri_lu_solve.hip", "getrs_small.hip", "api.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -41,7 +41,7 @@ def build(verbose=False, jobs=4):
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=jobs) as ex:
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]

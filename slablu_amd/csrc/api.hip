@@ -184,6 +184,43 @@ int slab_btri_solve(int dtype, int b, int m, int bs, const void* diag, int64_t d
   return -1;
 }
 
+int slab_btri_fused(int dtype, int bs) {
+  if (bs < 1) return 0;
+  if (dtype == SLAB_F64) return btri_lu_fused_ok<double>(bs) ? 1 : 0;
+  if (dtype == SLAB_F32) return btri_lu_fused_ok<float>(bs) ? 1 : 0;
+  return -1;
+}
+
+int slab_btri_factor_twisted(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride,
+                             void* sub, int64_t l_bstride, int64_t l_istride, void* sup, int64_t u_bstride,
+                             int64_t u_istride, int* ipiv, int64_t p_bstride, int64_t p_istride, int* info,
+                             void* stream) {
+  if (b < 0 || m < 1 || bs < 1 || slab_btri_fused(dtype, bs) != 1) return -1;
+  if (dtype == SLAB_F64)
+    return status(btri_lu_factor_twisted<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub,
+                                                 l_bstride, l_istride, (double*)sup, u_bstride, u_istride, ipiv,
+                                                 p_bstride, p_istride, info, S(stream)));
+  return status(btri_lu_factor_twisted<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
+                                              l_istride, (float*)sup, u_bstride, u_istride, ipiv, p_bstride,
+                                              p_istride, info, S(stream)));
+}
+
+int slab_btri_solve_twisted(int dtype, int b, int m, int bs, const void* diag, int64_t d_bstride,
+                            int64_t d_istride, const void* sub, int64_t l_bstride, int64_t l_istride,
+                            const void* sup, int64_t u_bstride, int64_t u_istride, const int* ipiv,
+                            int64_t p_bstride, int64_t p_istride, void* f, int64_t f_bstride, int64_t ldf, int nrhs,
+                            void* stream) {
+  if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs || slab_btri_fused(dtype, bs) != 1) return -1;
+  if (nrhs == 0 || b == 0) return 0;
+  if (dtype == SLAB_F64)
+    return status(btri_lu_solve_twisted<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub,
+                                                l_bstride, l_istride, (double*)sup, u_bstride, u_istride, ipiv,
+                                                p_bstride, p_istride, (double*)f, f_bstride, ldf, nrhs, S(stream)));
+  return status(btri_lu_solve_twisted<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
+                                             l_istride, (float*)sup, u_bstride, u_istride, ipiv, p_bstride,
+                                             p_istride, (float*)f, f_bstride, ldf, nrhs, S(stream)));
+}
+
 int slab_btri_inv_fused(int dtype, int bs) {
   if (dtype == SLAB_F64) return btri_inv_fused_ok<double>(bs) ? 1 : 0;
   if (dtype == SLAB_F32) return btri_inv_fused_ok<float>(bs) ? 1 : 0;

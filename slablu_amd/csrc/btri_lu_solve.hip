@@ -6,22 +6,28 @@ namespace slab {
 
 namespace {
 
-// One workgroup per (chain, NR-column tile) for both sweeps.  The sweep is a
-// fixed sequence of 4m - 2 stages, each with one block resident in LDS:
+// One workgroup per (chain, NR-column tile).  A full sweep is a fixed
+// sequence of 4m - 2 stages, each with one block resident in LDS:
 //   forward  DIAGL(0), SUB(0), DIAGL(1), ..., SUB(m-2), DIAGL(m-1)
 //   backward DIAGU(m-1), SUP(m-2), DIAGU(m-2), ..., SUP(0), DIAGU(0)
 // DIAGL(i): permute (composed pivots), unit-lower solve, store y_i;
 // SUB(i): x_{i+1} = f_{i+1} - Lfac_i y_i; DIAGU(i): upper solve, store x_i;
-// SUP(i): y_i - Ufac_i x_{i+1}.  While a stage computes, the next stage's
-// block (PF elements per thread; any rest is loaded at hand-over), its
-// right-hand-side rows and its pivots are already being loaded into
-// registers, so global latency is off the critical path.
-template <typename T, int RPT, int PF>
-__global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv,
-                                                       int64_t pbs, int64_t pis, T* f, int64_t fbs, int64_t ldf,
-                                                       int nrhs, int NR) {
+// SUP(i): y_i - Ufac_i x_{i+1}.  mode 1 runs the forward half only; mode 2
+// runs the backward half of a chain whose block m (past its end) already
+// holds its solution: SUP(m-1), DIAGU(m-1), ..., SUP(0), DIAGU(0).  While a
+// stage computes, the next stage's block (PF elements per thread; any rest
+// is loaded at hand-over), its right-hand-side rows and its pivots are
+// already being loaded into registers, so global latency is off the
+// critical path.  Workgroups past c0's chains take c1's chains.
+template <typename T, int NR, int PF>
+__global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, ChainSet<T> c1, int64_t ldf,
+                                                       int nrhs, int mode) {
+  // rows per thread in the block products, for the largest block the fused
+  // factorization accepts (112 rows in FP64, 160 in FP32)
+  constexpr int BSMAX = sizeof(T) == 8 ? 112 : 160;
+  constexpr int RPT = (BSMAX * NR + 255) / 256;
   extern __shared__ unsigned char smem_raw[];
-  const int nr = NR;
+  constexpr int nr = NR;
   const int ldx = nr == 1 ? 1 : nr + 1;
   T* Mt = reinterpret_cast<T*>(smem_raw);  // bs x (bs+1): one block at a time
   T* B0 = Mt + bs * (bs + 1);              // bs x ldx
@@ -32,29 +38,43 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
   const int tid = threadIdx.x;
   const int ld = bs + 1;
   const int lgnr = 31 - __clz(nr);
-  const int c0 = blockIdx.y * NR;
-  const int ncols = (nrhs - c0) < NR ? (nrhs - c0) : NR;
+  const int col0 = blockIdx.y * NR;
+  const int ncols = (nrhs - col0) < NR ? (nrhs - col0) : NR;
+  const int cm1 = ncols - 1;
   const int nn = bs * bs, nf = bs * nr;
   const float rbs = 1.0f / (float)bs;
-  const int nst = 4 * m - 2;
   auto Mat = [&](int r, int k) -> T { return Mt[r * ld + k]; };
-  // stage s -> kind (0 DIAGL, 1 SUB, 2 DIAGU, 3 SUP) and block index i
-  auto decode = [&](int s, int& i) -> int {
-    if (s < 2 * m - 1) {
-      if (s & 1) { i = (s - 1) >> 1; return 1; }
-      i = s >> 1;
-      return 0;
-    }
-    const int t = s - (2 * m - 1);
-    if (t & 1) { i = m - 1 - ((t + 1) >> 1); return 3; }
-    i = m - 1 - (t >> 1);
-    return 2;
-  };
-  for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
-    T* fb = f + b * fbs;
-    const int* pb = ipiv + b * pbs;
+  const int nt = c0.nchains + c1.nchains;
+  for (int g = blockIdx.x; g < nt; g += gridDim.x) {
+    const bool sec = g >= c0.nchains;
+    const ChainSet<T> a = sec ? c1 : c0;
+    const int64_t b = sec ? g - c0.nchains : g;
+    const int m = a.m;
+    const int nst = mode == 0 ? 4 * m - 2 : (mode == 1 ? 2 * m - 1 : 2 * m);
+    // stage s -> kind (0 DIAGL, 1 SUB, 2 DIAGU, 3 SUP) and block index i
+    auto decode = [&](int s, int& i) -> int {
+      if (mode == 2) {
+        if (s & 1) { i = m - 1 - ((s - 1) >> 1); return 2; }
+        i = m - 1 - (s >> 1);
+        return 3;
+      }
+      if (s < 2 * m - 1) {
+        if (s & 1) { i = (s - 1) >> 1; return 1; }
+        i = s >> 1;
+        return 0;
+      }
+      const int t = s - (2 * m - 1);
+      if (t & 1) { i = m - 1 - ((t + 1) >> 1); return 3; }
+      i = m - 1 - (t >> 1);
+      return 2;
+    };
+    // the last block's DIAGU of a full sweep reuses the resident DIAGL block
+    auto resident = [&](int k, int i) -> bool { return mode == 0 && k == 2 && i == m - 1; };
+    T* fb = a.f + b * a.fbs;
+    const int* pb = a.ipiv + b * a.pbs;
     T* Xc = B0;
     T* Y = B1;
+    auto frow = [&](int i, int r) -> T* { return fb + (int64_t)i * a.fis + (int64_t)r * ldf + col0; };
     auto block_src = [&](int k, int i) -> const T* {
       if (k == 1) return a.sub + b * a.lbs + (int64_t)i * a.lis;
       if (k == 3) return a.sup + b * a.ubs + (int64_t)i * a.uis;
@@ -71,12 +91,12 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
     auto prefetch = [&](int s) {
       int i;
       const int k = decode(s, i);
-      if (!(k == 2 && i == m - 1)) {
+      if (!resident(k, i)) {
         const T* src = block_src(k, i);
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int e = tid + 256 * u;
-          pf[u] = (e < nn) ? src[e] : T(0);
+          pf[u] = src[e < nn ? e : nn - 1];
         }
       }
       const int fi = f_row(k, i);
@@ -84,21 +104,23 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           const int e = tid + 256 * u, r = e >> lgnr, c = e & (nr - 1);
-          pfx[u] = (e < nf && c < ncols) ? fb[(int64_t)(fi * bs + r) * ldf + c0 + c] : T(0);
+          const T v = *(frow(fi, r < bs ? r : bs - 1) + (c < cm1 ? c : cm1));
+          pfx[u] = (e < nf && c < ncols) ? v : T(0);
         }
       }
       if (k == 0) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int r = tid + 256 * u;
-          pfp[u] = (r < bs) ? pb[(int64_t)i * pis + r] - 1 : 0;
+          const int v = pb[(int64_t)i * a.pis + (r < bs ? r : bs - 1)];
+          pfp[u] = v - 1;
         }
       }
     };
     auto commit = [&](int s) {
       int i;
       const int k = decode(s, i);
-      if (!(k == 2 && i == m - 1)) {
+      if (!resident(k, i)) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int e = tid + 256 * u;
@@ -125,9 +147,16 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
     auto store_f = [&](int i, const T* Z) {
       for (int e = tid; e < nf; e += 256) {
         const int r = e >> lgnr, c = e & (nr - 1);
-        if (c < ncols) fb[(int64_t)(i * bs + r) * ldf + c0 + c] = Z[r * ldx + c];
+        if (c < ncols) *(frow(i, r) + c) = Z[r * ldx + c];
       }
     };
+    if (mode == 2) {
+      // the block past the chain end holds its solution: x_m
+      for (int e = tid; e < nf; e += 256) {
+        const int r = e >> lgnr, c = e & (nr - 1);
+        Y[r * ldx + c] = (c < ncols) ? *(frow(m, r) + c) : T(0);
+      }
+    }
     prefetch(0);
     commit(0);
     __syncthreads();
@@ -142,18 +171,19 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
           Y[r * ldx + c] = Xc[Q[r] * ldx + c];
         }
         __syncthreads();
-        tri_solve<T, false, true>(Mat, bs, Y, ldx, nr);
+        tri_solve_n<T, false, true, NR>(Mat, bs, Y, ldx);
         store_f(i, Y);
       } else if (k == 1 || k == 3) {
-        if (nr == 1)
+        if constexpr (NR == 1)
           sub_mat_vec1<T>(Mt, Y, Xc, bs, W);
         else
           sub_mat_vec<T, RPT>(Mt, Y, Xc, bs, nr, ldx);
       } else {
-        T* Z = (i == m - 1) ? Y : Xc;
-        tri_solve<T, true, false>(Mat, bs, Z, ldx, nr);
+        const bool own = mode == 0 && i == m - 1;
+        T* Z = own ? Y : Xc;
+        tri_solve_n<T, true, false, NR>(Mat, bs, Z, ldx);
         store_f(i, Z);
-        if (i != m - 1) {
+        if (!own) {
           T* t = Xc;
           Xc = Y;
           Y = t;
@@ -166,25 +196,45 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int nb, int m, int bs, Bt
   }
 }
 
-template <typename T, int RPT, int PF>
-hipError_t launch_lu_solve(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv, int64_t pbs, int64_t pis, T* f,
-                           int64_t fbs, int64_t ldf, int nrhs, int nr, hipStream_t s) {
-  const size_t lds = lu_solve_lds<T>(bs, nr);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_solve<T, RPT, PF>),
+template <typename T, int NR, int PF>
+hipError_t launch_lu_solve(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
+                           hipStream_t s) {
+  const size_t lds = lu_solve_lds<T>(bs, NR);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  dim3 grid(nb < 65535 ? nb : 65535, (nrhs + nr - 1) / nr);
-  hipLaunchKernelGGL((k_btri_lu_solve<T, RPT, PF>), grid, dim3(256), lds, s, nb, m, bs, a, ipiv, pbs, pis, f, fbs,
-                     ldf, nrhs, nr);
+  const int nt = c0.nchains + c1.nchains;
+  if (nt <= 0) return hipSuccess;
+  dim3 grid(nt < 65535 ? nt : 65535, (nrhs + NR - 1) / NR);
+  hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
   return hipGetLastError();
 }
 
-template <typename T, int RPT>
-hipError_t launch_lu_solve_pf(int nb, int m, int bs, BtriArgs<T> a, const int* ipiv, int64_t pbs, int64_t pis,
-                              T* f, int64_t fbs, int64_t ldf, int nrhs, int nr, hipStream_t s) {
-  if ((int64_t)bs * bs <= 256 * 8)
-    return launch_lu_solve<T, RPT, 8>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  return launch_lu_solve<T, RPT, 40>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+// PF: block elements prefetched per thread (the rest of a block is loaded
+// at hand-over); FP64 with 16 or 32 columns keeps it small to stay within
+// the register file
+template <typename T, int NR>
+hipError_t launch_lu_solve_pf(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
+                              hipStream_t s) {
+  constexpr int PFBIG = sizeof(T) == 8 ? (NR >= 32 ? 8 : (NR >= 16 ? 16 : 40)) : 40;
+  if ((int64_t)bs * bs <= 256 * 8) return launch_lu_solve<T, NR, 8>(bs, c0, c1, ldf, nrhs, mode, s);
+  return launch_lu_solve<T, NR, PFBIG>(bs, c0, c1, ldf, nrhs, mode, s);
+}
+
+template <typename T>
+hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
+                      hipStream_t s) {
+  int nr = nrhs >= 32 ? 32 : (nrhs >= 16 ? 16 : (nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1))));
+  while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
+  if (lu_solve_lds<T>(bs, nr) > kLdsMax || bs > (sizeof(T) == 8 ? 112 : 160)) return hipErrorInvalidValue;
+  switch (nr) {
+    case 1: return launch_lu_solve_pf<T, 1>(bs, c0, c1, ldf, nrhs, mode, s);
+    case 2: return launch_lu_solve_pf<T, 2>(bs, c0, c1, ldf, nrhs, mode, s);
+    case 4: return launch_lu_solve_pf<T, 4>(bs, c0, c1, ldf, nrhs, mode, s);
+    case 8: return launch_lu_solve_pf<T, 8>(bs, c0, c1, ldf, nrhs, mode, s);
+    case 16: return launch_lu_solve_pf<T, 16>(bs, c0, c1, ldf, nrhs, mode, s);
+    default: return launch_lu_solve_pf<T, 32>(bs, c0, c1, ldf, nrhs, mode, s);
+  }
 }
 
 }  // namespace
@@ -193,23 +243,55 @@ template <typename T>
 hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
                                int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s) {
-  BtriArgs<T> a{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis};
-  int nr = nrhs >= 32 ? 32 : (nrhs >= 16 ? 16 : (nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1))));
-  while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
-  const int rpt = (bs + 256 / nr - 1) / (256 / nr);
-  if (lu_solve_lds<T>(bs, nr) > kLdsMax || rpt > 24 || bs > 512) return hipErrorInvalidValue;
-  if (rpt <= 1) return launch_lu_solve_pf<T, 1>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  if (rpt <= 2) return launch_lu_solve_pf<T, 2>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  if (rpt <= 4) return launch_lu_solve_pf<T, 4>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  if (rpt <= 8) return launch_lu_solve_pf<T, 8>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  if (rpt <= 13) return launch_lu_solve_pf<T, 13>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
-  return launch_lu_solve_pf<T, 24>(nb, m, bs, a, ipiv, pbs, pis, f, fbs, ldf, nrhs, nr, s);
+  ChainSet<T> c0{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, const_cast<int*>(ipiv), pbs, pis, nullptr,
+                 f, fbs, (int64_t)bs * ldf, m, nb, 0};
+  ChainSet<T> none{};
+  return solve_sets<T>(bs, c0, none, ldf, nrhs, 0, s);
+}
+
+// Sweeps with the factors of btri_lu_factor_twisted: forward down both
+// chains, the middle block's right-hand side less both chains' couplings,
+// the middle block, then backward out along both chains.
+template <typename T>
+hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                                 int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
+                                 int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s) {
+  const int k = m / 2, lb = m - 1 - k;
+  int* iv = const_cast<int*>(ipiv);
+  const int64_t fis = (int64_t)bs * ldf;
+  ChainSet<T> none{};
+  ChainSet<T> top{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv, pbs, pis, nullptr, f, fbs, fis, k, nb, 1};
+  ChainSet<T> bot = mirrored<T>(nb, m, lb, bs, diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv, pbs, pis, nullptr,
+                                f, fbs, ldf);
+  hipError_t e;
+  if (k > 0) {
+    if ((e = solve_sets<T>(bs, top, bot, ldf, nrhs, 1, s)) != hipSuccess) return e;
+    View<T> Fk{f + (int64_t)k * fis, ldf, 1, fbs};
+    View<T> Lt{sub + (int64_t)(k - 1) * lis, bs, 1, lbs};
+    if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Lt, View<T>{f + (int64_t)(k - 1) * fis, ldf, 1, fbs}, T(1), Fk, s)) !=
+        hipSuccess)
+      return e;
+    if (lb > 0) {
+      View<T> Lb{sup + (int64_t)k * uis, bs, 1, ubs};
+      if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Lb, View<T>{f + (int64_t)(k + 1) * fis, ldf, 1, fbs}, T(1), Fk,
+                       s)) != hipSuccess)
+        return e;
+    }
+  }
+  ChainSet<T> mid{diag + (int64_t)k * dis, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv + (int64_t)k * pis, pbs, pis,
+                  nullptr, f + (int64_t)k * fis, fbs, fis, 1, nb, 0};
+  if ((e = solve_sets<T>(bs, mid, none, ldf, nrhs, 0, s)) != hipSuccess) return e;
+  if (k > 0) return solve_sets<T>(bs, top, bot, ldf, nrhs, 2, s);
+  return hipSuccess;
 }
 
 #define SLAB_BTRI_LU_SOLVE_INST(T)                                                                                \
   template hipError_t btri_lu_solve_fused<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,       \
                                              int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t, int64_t, \
-                                             int, hipStream_t);
+                                             int, hipStream_t);                                                    \
+  template hipError_t btri_lu_solve_twisted<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,     \
+                                               int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t,        \
+                                               int64_t, int, hipStream_t);
 SLAB_BTRI_LU_SOLVE_INST(double)
 SLAB_BTRI_LU_SOLVE_INST(float)
 

@@ -25,6 +25,42 @@ struct BtriArgs {
   T* sup; int64_t ubs, uis;
 };
 
+// One set of chains for the fused LU-form kernels: base pointers with the
+// stride between chains (*bs) and between consecutive blocks of a chain
+// (*is; negative to walk a chain backwards), m blocks per chain.  sub_i
+// couples block i+1 to block i and sup_i block i to block i+1 in chain
+// order; open: the factorization also forms Ufac/Lfac of the last block
+// (its couplings reach a block past the end of the chain).
+template <typename T>
+struct ChainSet {
+  T* diag; int64_t dbs, dis;
+  T* sub; int64_t lbs, lis;
+  T* sup; int64_t ubs, uis;
+  int* ipiv; int64_t pbs, pis;
+  int* info;
+  T* f; int64_t fbs, fis;
+  int m, nchains, open;
+};
+
+// The chain set that walks blocks m-1 .. m-len of each chain backwards:
+// diag and pivots mirrored, sub'_t = sup_{m-2-t}, sup'_t = sub_{m-2-t}.
+template <typename T>
+inline ChainSet<T> mirrored(int nb, int m, int len, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
+                     int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis, int* info,
+                     T* f, int64_t fbs, int64_t ldf) {
+  ChainSet<T> c{};
+  c.diag = diag + (int64_t)(m - 1) * dis; c.dbs = dbs; c.dis = -dis;
+  c.sub = sup + (int64_t)(m - 2) * uis; c.lbs = ubs; c.lis = -uis;
+  c.sup = sub + (int64_t)(m - 2) * lis; c.ubs = lbs; c.uis = -lis;
+  c.ipiv = ipiv + (int64_t)(m - 1) * pis; c.pbs = pbs; c.pis = -pis;
+  c.info = info;
+  if (f != nullptr) {
+    c.f = f + (int64_t)(m - 1) * bs * ldf; c.fbs = fbs; c.fis = -(int64_t)bs * ldf;
+  }
+  c.m = len; c.nchains = len > 0 ? nb : 0; c.open = 1;
+  return c;
+}
+
 // acc[a][q] += sum_k A(r_a, k) B(k, c_q) for bs x bs row-major A and B in
 // global memory; thread (ty, tx) of a 16 x 16 arrangement owns rows
 // ty + 16a and columns tx + 16q (a, q < RT).  K runs in KC-wide chunks
@@ -315,6 +351,32 @@ __device__ inline void tri_solve(At at, int n, T* X, int ldx, int nc) {
         }
         break;
     }
+  }
+}
+
+// tri_solve for a compile-time column count NC (1, 2, 4, 8, 16 or 32): only
+// the one variant is instantiated.
+template <typename T, bool UPPER, bool UNIT, int NC, typename At>
+__device__ inline void tri_solve_n(At at, int n, T* X, int ldx) {
+  if constexpr (NC <= 8)
+    tri_solve_wave<T, UPPER, UNIT, NC>(at, n, X, ldx, NC);
+  else
+    tri_solve_tiled<T, UPPER, UNIT, NC / 16>(at, n, X, ldx, NC);
+}
+
+// Tiled triangular solve for any nc: chunks of 16 CQM columns, the last one
+// padded to 16 times a power of two columns, so ldx must be at least
+// lu_factor_ldx<T>(nc) - 1 (columns past nc are scratch).
+template <typename T, bool UPPER, bool UNIT, typename At>
+__device__ inline void tri_solve_wide(At at, int n, T* X, int ldx, int nc) {
+  constexpr int CQM = sizeof(T) == 8 ? 4 : 8;
+  for (int c0 = 0; c0 < nc; c0 += 16 * CQM) {
+    const int w = (nc - c0) < 16 * CQM ? (nc - c0) : 16 * CQM;
+    const int cq = (w + 15) >> 4;
+    if (cq == 1) tri_solve_tiled<T, UPPER, UNIT, 1>(at, n, X + c0, ldx, w);
+    else if (cq == 2) tri_solve_tiled<T, UPPER, UNIT, 2>(at, n, X + c0, ldx, w);
+    else if (cq <= 4) tri_solve_tiled<T, UPPER, UNIT, 4>(at, n, X + c0, ldx, w);
+    else if constexpr (CQM > 4) tri_solve_tiled<T, UPPER, UNIT, CQM>(at, n, X + c0, ldx, w);
   }
 }
 
@@ -633,14 +695,23 @@ constexpr size_t kLdsMax = 160 * 1024;
 // and Gauss-Jordan kernels, which pad blocks to 16 RT).
 __host__ __device__ inline int tile_rt(int bs) { return (bs + 15) / 16; }
 
-// row stride of the right-hand-side chunk (xc columns rounded up to 16 for
-// the tiled triangular solves, plus one)
-__host__ __device__ inline int lu_factor_ldx(int xc) { return ((xc + 15) & ~15) + 1; }
+// row stride of the factor kernel's right-hand-side chunk: tri_solve_wide
+// runs full chunks of 16 CQM columns and pads the last one to 16 times a
+// power of two columns; plus one for odd strides
+template <typename T>
+__host__ __device__ inline int lu_factor_ldx(int xc) {
+  constexpr int CQM = sizeof(T) == 8 ? 4 : 8;
+  const int full = (xc / (16 * CQM)) * (16 * CQM);
+  const int q = (xc - full + 15) / 16;
+  int p2 = q > 0 ? 1 : 0;
+  while (p2 < q) p2 <<= 1;
+  return full + 16 * p2 + 1;
+}
 
 template <typename T>
 size_t lu_factor_lds(int bs, int xc) {
   const size_t bp = 16 * (size_t)tile_rt(bs);
-  return sizeof(T) * (bp * (bp + 1) + (size_t)bs * lu_factor_ldx(xc) + bp * (KC + 1) + (size_t)KC * (bp + 1)) +
+  return sizeof(T) * (bp * (bp + 1) + (size_t)bs * lu_factor_ldx<T>(xc) + bp * (KC + 1) + (size_t)KC * (bp + 1)) +
          2 * sizeof(int) * bs;
 }
 

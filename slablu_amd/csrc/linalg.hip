@@ -819,7 +819,9 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
   // a single tall matrix: one row per thread in 1024-thread workgroups
   if (nrows <= 256) return getrf_reg<T, 32, 1, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   if (nrows <= 512) return getrf_reg<T, 32, 2, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 1024) return getrf_reg<T, 16, 1, 1024>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  // (8 FP64 or 16 FP32 columns per panel keep 1024 threads within 128 VGPRs)
+  if (nrows <= 1024)
+    return getrf_reg<T, sizeof(T) == 8 ? 8 : 16, 1, 1024>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   if (nrows <= 2048) return getrf_reg<T, 8, 8, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   // taller panels: LDS-resident panel, generic interchanges
   int nb = 32;

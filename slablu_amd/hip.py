@@ -40,6 +40,11 @@ def _load():
         "slab_btri_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                              _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
         "slab_btri_inv_fused": ([_int, _int], _int),
+        "slab_btri_fused": ([_int, _int], _int),
+        "slab_btri_factor_twisted": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                                      _i64, _i64, _vp, _vp], _int),
+        "slab_btri_solve_twisted": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                                     _i64, _i64, _vp, _i64, _i64, _int, _vp], _int),
         "slab_btri_inv_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                                   _vp, _vp, _vp], _int),
         "slab_btri_inv_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
@@ -168,27 +173,40 @@ def getrs(LU, ipiv, B, trans=False):
     return B
 
 
-def btri_factor(diag, sub, sup, ipiv, info=None):
+def btri_fused(dtype, bs):
+    """True when bs x bs blocks run in the fused LU-form chain kernels (the
+    twisted factorization requires it)."""
+    return LIB.slab_btri_fused(0 if dtype == torch.float64 else 1, int(bs)) == 1
+
+
+def btri_factor(diag, sub, sup, ipiv, info=None, twisted=False):
     """Block-tridiagonal LU in place: diag (b, m, bs, bs), sub/sup (b, m-1, bs,
-    bs), ipiv (b, m, bs) int32; blocks must be contiguous row-major."""
+    bs), ipiv (b, m, bs) int32; blocks must be contiguous row-major.
+    twisted: eliminate from both chain ends towards block m//2 (factors then
+    serve btri_solve(..., twisted=True) only)."""
     b, m, bs, _ = diag.shape
     for t in (diag, sub, sup):
         _require(t.stride(-1) == 1 and t.stride(-2) == bs, "btri blocks must be contiguous row-major")
     _require(ipiv.stride(-1) == 1, "ipiv layout")
-    _check(LIB.slab_btri_factor(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub),
-                                sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
-                                sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(info),
-                                _stream()), "slab_btri_factor")
+    fn = LIB.slab_btri_factor_twisted if twisted else LIB.slab_btri_factor
+    _check(fn(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub),
+              sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
+              sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(info),
+              _stream()), "slab_btri_factor_twisted" if twisted else "slab_btri_factor")
 
 
-def btri_solve(diag, sub, sup, ipiv, f, transpose=False):
+def btri_solve(diag, sub, sup, ipiv, f, transpose=False, twisted=False):
     """In-place sweeps on f (b, m*bs, nrhs) (row-major rows)."""
     b, m, bs, _ = diag.shape
     _require(f.dim() == 3 and f.shape[0] == b and f.shape[1] == m * bs and f.stride(2) == 1, "btri rhs layout")
-    _check(LIB.slab_btri_solve(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub),
-                               sub.stride(0), sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0),
-                               sup.stride(1) if m > 1 else 0, _p(ipiv), ipiv.stride(0), ipiv.stride(1), _p(f),
-                               f.stride(0), f.stride(1), f.shape[2], int(transpose), _stream()), "slab_btri_solve")
+    args = (_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub), sub.stride(0),
+            sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0), sup.stride(1) if m > 1 else 0, _p(ipiv),
+            ipiv.stride(0), ipiv.stride(1), _p(f), f.stride(0), f.stride(1), f.shape[2])
+    if twisted:
+        _require(not transpose, "twisted factors have no transposed sweep")
+        _check(LIB.slab_btri_solve_twisted(*args, _stream()), "slab_btri_solve_twisted")
+    else:
+        _check(LIB.slab_btri_solve(*args, int(transpose), _stream()), "slab_btri_solve")
     return f
 
 

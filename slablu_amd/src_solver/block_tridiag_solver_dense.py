@@ -44,11 +44,17 @@ def unpack_pivs(b, m, bs, Tdata, device=None):
 
 
 class Block_Tridiagonal:
-    def __init__(self, b, m, bs, form="lu"):
+    def __init__(self, b, m, bs, form="lu", twisted=False):
+        """form 'lu': the reference's block LU (packed LU + Lfac/Ufac + pivots
+        in Tdata); 'inv': block-Thomas with explicit diagonal inverses.
+        twisted (form 'lu'): eliminate from both ends of each chain towards
+        block m//2, which halves the dependent steps of the factorization and
+        of every sweep; Tdata then holds the twisted factors (same layout)."""
         if form not in ("inv", "lu"):
             raise ValueError("form must be 'inv' or 'lu'")
         self.b, self.m, self.bs = b, m, bs
         self.form = form
+        self.twisted = bool(twisted) and form == "lu"
         self.pivdata_nblocks = int(m / bs) + 1
         self.storage_blocks = m + 2 * (m - 1) + self.pivdata_nblocks
         self.nbytes = (self.b * self.storage_blocks * self.bs ** 2) * 8
@@ -85,7 +91,9 @@ class Block_Tridiagonal:
             hip.btri_inv_factor(d, l, u, info=self.info)
             return
         self.pivs = torch.zeros(b, m, bs, dtype=torch.int32, device=self.Tdata.device)
-        hip.btri_factor(d, l, u, self.pivs, info=self.info)
+        if self.twisted and not hip.btri_fused(self.Tdata.dtype, bs):
+            self.twisted = False
+        hip.btri_factor(d, l, u, self.pivs, info=self.info, twisted=self.twisted)
         pack_pivs(b, m, bs, self.Tdata, self.pivs)
 
     def lusolve_sweep(self, f, b_start=None, b_end=None, transpose=False):
@@ -103,7 +111,9 @@ class Block_Tridiagonal:
                 raise NotImplementedError("transposed sweeps need form='lu'")
             hip.btri_inv_solve(d, l, u, fv)
         else:
-            hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose)
+            if transpose and self.twisted:
+                raise NotImplementedError("transposed sweeps need the non-twisted factorization")
+            hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose, twisted=self.twisted)
         return fv
 
 

@@ -278,7 +278,7 @@ class Domain_Solver:
         diag = Arr_d - CW[:, :mr, 1, :, 1, :] - CW[:, 1:, 0, :, 0, :]
         sup = Arr_u - CW[:, 1:mr, 0, :, 1, :]
         sub = Arr_l - CW[:, 1:mr, 1, :, 0, :]
-        self.Ared_solver = Block_Tridiagonal(Np, mr, B)
+        self.Ared_solver = Block_Tridiagonal(Np, mr, B, twisted=True)
         self.Ared_solver.build_sweep(diag.contiguous(), sub.contiguous(), sup.contiguous())
 
     def _slab_u_local(self, upad, s):

@@ -15,7 +15,8 @@ def _declared():
 def test_header_declares_entry_points():
     names = _declared()
     for must in ("slab_gemm", "slab_getrf", "slab_btri_factor", "slab_btri_solve", "slab_btri_inv_factor",
-                 "slab_btri_inv_solve", "slab_leaf_assemble"):
+                 "slab_btri_inv_solve", "slab_leaf_assemble", "slab_btri_factor_twisted", "slab_btri_solve_twisted",
+                 "slab_btri_fused"):
         assert must in names
 
 
@@ -45,3 +46,11 @@ def test_bad_arguments_are_rejected_without_a_gpu():
     assert lib.slab_getrf(*args) == -1  # nelim > npiv
     assert lib.slab_btri_inv_fused(0, 100) == 1 and lib.slab_btri_inv_fused(0, 1000) == 0
     assert lib.slab_btri_inv_fused(7, 10) == -1
+    assert lib.slab_btri_fused(1, 100) == 1 and lib.slab_btri_fused(0, 100) == 1
+    assert lib.slab_btri_fused(0, 1000) == 0 and lib.slab_btri_fused(3, 10) == -1
+    # twisted factorization needs blocks that fit the fused kernels
+    lib.slab_btri_factor_twisted.restype = ctypes.c_int
+    targs = [ctypes.c_int(0), ctypes.c_int(1), ctypes.c_int(3), ctypes.c_int(1000)] + \
+        [ctypes.c_void_p(0), ctypes.c_int64(0), ctypes.c_int64(0)] * 3 + \
+        [ctypes.c_void_p(0), ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_void_p(0), ctypes.c_void_p(0)]
+    assert lib.slab_btri_factor_twisted(*targs) == -1

@@ -175,9 +175,14 @@ def _btri_dense(diag, sub, sup):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (1, 4, 100), (4, 2, 37)])
+@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (1, 4, 100), (4, 2, 37), (2, 3, 24), (2, 11, 100)])
 @pytest.mark.parametrize("transpose", [False, True])
-def test_btri_factor_solve(dtype, b, m, bs, transpose):
+@pytest.mark.parametrize("twisted", [False, True])
+def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
+    """Block-tridiagonal LU factor + sweeps (sequential, or twisted: both
+    chain ends towards block m//2) against a dense solve."""
+    if twisted and transpose:
+        pytest.skip("twisted factors have no transposed sweep")
     hip = _hip()
     g = torch.Generator().manual_seed(b * 100 + m * 10 + bs)
     diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
@@ -187,9 +192,9 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose):
     f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
     diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
     ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
-    hip.btri_factor(diag, sub, sup, ipiv)
+    hip.btri_factor(diag, sub, sup, ipiv, twisted=twisted)
     f = f0.to("cuda", dtype).contiguous()
-    hip.btri_solve(diag, sub, sup, ipiv, f, transpose=transpose)
+    hip.btri_solve(diag, sub, sup, ipiv, f, transpose=transpose, twisted=twisted)
     M = _btri_dense(diag0, sub0[:, :m - 1], sup0[:, :m - 1])
     ref = torch.linalg.solve(M.transpose(1, 2) if transpose else M, f0)
     torch.cuda.synchronize()
@@ -281,7 +286,8 @@ def test_getrs_small_rhs(dtype, n, nrhs):
 
 
 @pytest.mark.parametrize("form", ["inv", "lu"])
-def test_block_tridiagonal_api(form):
+@pytest.mark.parametrize("twisted", [False, True])
+def test_block_tridiagonal_api(form, twisted):
     """Block_Tridiagonal(b, m, bs) drop-in: build_sweep / lusolve_sweep on
     slices of the batch, as slablu_solver.py uses it."""
     from slablu_amd.src_solver.block_tridiag_solver_dense import Block_Tridiagonal
@@ -290,7 +296,7 @@ def test_block_tridiagonal_api(form):
     diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 4 * torch.eye(bs, dtype=torch.float64)
     sub0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
     sup0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
-    T = Block_Tridiagonal(b, m, bs, form=form)
+    T = Block_Tridiagonal(b, m, bs, form=form, twisted=twisted)
     T.build_sweep(diag0.cuda(), sub0.cuda(), sup0.cuda())
     assert T.Tdata.shape == (b, T.storage_blocks, bs, bs)
     f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
@@ -298,7 +304,7 @@ def test_block_tridiagonal_api(form):
     x = T.lusolve_sweep(f0[2:5].cuda().clone(), b_start=2, b_end=5)
     torch.cuda.synchronize()
     assert _rel(x, torch.linalg.solve(M[2:5], f0[2:5])) < 1e-11
-    if form == "lu":
+    if form == "lu" and not T.twisted:
         xt = T.lusolve_sweep(f0.cuda().clone(), transpose=True)
         torch.cuda.synchronize()
         assert _rel(xt, torch.linalg.solve(M.transpose(1, 2), f0)) < 1e-11
@@ -322,3 +328,26 @@ def test_hbs_block_tridiagonal_api(nb):
     M = _btri_dense((d0 + d1)[None], sub[None], sup[None])[0]
     torch.cuda.synchronize()
     assert _rel(x, torch.linalg.solve(M, f)) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nrhs", [1, 2, 16, 40])
+def test_btri_twisted_rem_shape(dtype, nrhs):
+    """Twisted factor + sweeps at the slab-chain shape (bs = 100, odd and
+    even chain lengths) for single, few and many right-hand sides."""
+    hip = _hip()
+    for b, m in ((2, 9), (1, 8)):
+        g = torch.Generator().manual_seed(31 * m + nrhs)
+        bs = 100
+        diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
+        sub0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+        sup0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+        f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
+        diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
+        ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
+        hip.btri_factor(diag, sub, sup, ipiv, twisted=True)
+        f = f0.to("cuda", dtype).contiguous()
+        hip.btri_solve(diag, sub, sup, ipiv, f, twisted=True)
+        ref = torch.linalg.solve(_btri_dense(diag0, sub0, sup0), f0)
+        torch.cuda.synchronize()
+        assert _rel(f, ref) < 200 * TOL[dtype], (b, m)
