@@ -65,6 +65,18 @@ int slab_getrs(int dtype, int batch, int n, int nrhs, int trans,
                const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv, int64_t p_bs,
                void* B, int64_t ldb, int64_t b_bs, void* stream);
 
+/* perm[r] (0-based) = the row of B that the interchanges ipiv (1-based, as
+ * slab_getrf writes them) bring to row r; batched, perm row stride q_bs.
+ * Lets repeated solves with one factorization skip replaying the swaps. */
+int slab_piv_to_perm(int batch, int n, const int* ipiv, int64_t p_bs, int* perm, int64_t q_bs, void* stream);
+
+/* slab_getrs (non-transposed, 1 <= nrhs <= 8) with the permutation of
+ * slab_piv_to_perm, one wavefront per matrix: the leaf interior solves
+ * A_cc^{-1} f (torch.linalg.solve in src_disc/hps_parallel_leaf_ops.py:86-100)
+ * with the factors kept from the leaf condensation. */
+int slab_getrs_perm(int dtype, int batch, int n, int nrhs, const void* LU, int64_t lda, int64_t lu_bs,
+                    const int* perm, int64_t q_bs, void* B, int64_t ldb, int64_t b_bs, void* stream);
+
 /* Block-tridiagonal LU without inter-block pivoting over b independent
  * chains of m blocks of size bs (row-major bs x bs blocks).  Overwrites
  * diag_i with the packed LU of D_i - Lfac_i Ufac_i, sub_i with

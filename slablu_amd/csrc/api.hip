@@ -143,6 +143,24 @@ int slab_getrf(int dtype, int batch, int nrows, int ncols, int nelim, int npiv, 
   return -1;
 }
 
+int slab_piv_to_perm(int batch, int n, const int* ipiv, int64_t p_bs, int* perm, int64_t q_bs, void* stream) {
+  if (batch < 0 || n < 0) return -1;
+  return status(piv_to_perm_batched(batch, n, ipiv, p_bs, perm, q_bs, S(stream)));
+}
+
+int slab_getrs_perm(int dtype, int batch, int n, int nrhs, const void* LU, int64_t lda, int64_t lu_bs,
+                    const int* perm, int64_t q_bs, void* B, int64_t ldb, int64_t b_bs, void* stream) {
+  if (batch < 0 || n < 1 || nrhs < 1 || nrhs > 8) return -1;
+  if (batch == 0) return 0;
+  if (dtype == SLAB_F64)
+    return status(getrs_perm<double>(batch, n, nrhs, view<double>(const_cast<void*>(LU), lda, 1, lu_bs), perm, q_bs,
+                                     view<double>(B, ldb, 1, b_bs), S(stream)));
+  if (dtype == SLAB_F32)
+    return status(getrs_perm<float>(batch, n, nrhs, view<float>(const_cast<void*>(LU), lda, 1, lu_bs), perm, q_bs,
+                                    view<float>(B, ldb, 1, b_bs), S(stream)));
+  return -1;
+}
+
 int slab_getrs(int dtype, int batch, int n, int nrhs, int trans, const void* LU, int64_t lda, int64_t lu_bs,
                const int* ipiv, int64_t p_bs, void* B, int64_t ldb, int64_t b_bs, void* stream) {
   if (batch < 0 || n < 0 || nrhs < 0) return -1;

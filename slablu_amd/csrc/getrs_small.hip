@@ -109,6 +109,143 @@ __global__ __launch_bounds__(256) void k_getrs_small(int batch, int n, int nrhs,
   }
 }
 
+// Row segment row[0..63] of a factor (entries past tb read as zero) with
+// 16-byte loads when the rows are aligned.
+template <typename T>
+__device__ inline void load_seg(const T* row, bool vec, int tb, T (&a)[64]) {
+  constexpr int V = 16 / sizeof(T);
+  if (vec && tb == 64) {
+    typedef T vec_t __attribute__((ext_vector_type(V)));
+    const vec_t* rv = reinterpret_cast<const vec_t*>(row);
+#pragma unroll
+    for (int q = 0; q < 64 / V; ++q) {
+      const vec_t v = rv[q];
+#pragma unroll
+      for (int u = 0; u < V; ++u) a[q * V + u] = v[u];
+    }
+  } else {
+    const int tm1 = tb - 1;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+      const T v = row[j < tm1 ? j : tm1];
+      a[j] = j < tb ? v : T(0);
+    }
+  }
+}
+
+// Batched LU solve with the composed row permutation (perm[r]: the row of B
+// that P B puts at row r) for up to NC right-hand sides, one wavefront per
+// matrix (four per workgroup) and no workgroup barriers.  Per 64-row
+// diagonal block each lane holds its row of the block in registers and the
+// block is solved by substitution with lane = row and readlane broadcasts;
+// then each lane takes rows below/above the block, streaming their 64-entry
+// segments of LU with independent 16-byte loads against the block's solved
+// values (LDS broadcasts).  With one wavefront per matrix, thousands of
+// matrices are in flight and their substitution and memory phases overlap.
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_getrs_wave(int batch, int n, int nrhs, View<T> LU, const int* perm,
+                                                    int64_t qbs, View<T> B) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T* X = reinterpret_cast<T*>(smem_raw) + (size_t)wave * n * NC;  // n x NC
+  const int b = blockIdx.x * 4 + wave;
+  if (b >= batch) return;
+  constexpr int V = 16 / sizeof(T);
+  const T* base = LU.at(b, 0, 0);
+  const bool vec = LU.cs == 1 && (LU.rs % V) == 0 && (reinterpret_cast<uintptr_t>(base) % 16) == 0;
+  const int* pq = perm + (int64_t)b * qbs;
+  const int cm1 = nrhs - 1;
+  for (int r = lane; r < n; r += 64) {
+    const int src = pq[r];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const T v = *B.at(b, src, c < cm1 ? c : cm1);
+      X[r * NC + c] = c < nrhs ? v : T(0);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int nblk = (n + 63) >> 6;
+  for (int q = 0; q < 2 * nblk; ++q) {
+    const bool lower = q < nblk;
+    const int d0 = (lower ? q : 2 * nblk - 1 - q) * 64;
+    const int tb = (n - d0) < 64 ? (n - d0) : 64;
+    const bool act = lane < tb;
+    const int rl = d0 + (act ? lane : tb - 1);
+    T a[64];
+    load_seg<T>(LU.at(b, rl, d0), vec, tb, a);
+    T x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = X[rl * NC + c];
+    if (lower) {
+      static_for<0, 64>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if (j < tb) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const T xj = rdlane(x[c], j);
+            x[c] = (lane > j) ? x[c] - a[j] * xj : x[c];
+          }
+        }
+      });
+    } else {
+      const T rinv = T(1) / *LU.at(b, rl, rl);
+      static_for<0, 64>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = 63 - decltype(jc)::value;
+        if (j < tb) {
+          const T rj = rdlane(rinv, j);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const T xj = rdlane(x[c], j) * rj;
+            x[c] = (lane == j) ? xj : ((lane < j) ? x[c] - a[j] * xj : x[c]);
+          }
+        }
+      });
+    }
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) X[(d0 + lane) * NC + c] = x[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int r0 = lower ? d0 + tb : 0;
+    const int r1 = lower ? n : d0;
+    const T* Xb = X + d0 * NC;
+    for (int r = r0 + lane; r < r1; r += 64) {
+      T l[64];
+      load_seg<T>(LU.at(b, r, d0), vec, tb, l);
+      T acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = T(0);
+#pragma unroll
+      for (int j = 0; j < 64; ++j)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += l[j] * Xb[(j < tb ? j : tb - 1) * NC + c];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) X[r * NC + c] -= acc[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int r = lane; r < n; r += 64)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (c < nrhs) *B.at(b, r, c) = X[r * NC + c];
+}
+
+// perm[r] = the row that the interchanges ipiv (1-based, as getrf) bring to
+// row r, one workgroup per matrix.
+__global__ __launch_bounds__(256) void k_piv_to_perm(int batch, int n, const int* ipiv, int64_t pbs, int* perm,
+                                                     int64_t qbs) {
+  extern __shared__ unsigned char smem_raw[];
+  int* P = reinterpret_cast<int*>(smem_raw);
+  int* Q = P + n;
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    for (int r = threadIdx.x; r < n; r += 256) P[r] = ipiv[(int64_t)b * pbs + r] - 1;
+    __syncthreads();
+    piv_to_perm(P, n, Q);
+    for (int r = threadIdx.x; r < n; r += 256) perm[(int64_t)b * qbs + r] = Q[r];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 template <typename T>
@@ -125,6 +262,43 @@ hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, 
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t getrs_perm(int batch, int n, int nrhs, View<T> LU, const int* perm, int64_t qbs, View<T> B,
+                      hipStream_t s) {
+  if (nrhs > 8 || nrhs < 1 || n < 1) return hipErrorInvalidValue;
+  const int nc = nrhs == 1 ? 1 : (nrhs <= 2 ? 2 : (nrhs <= 4 ? 4 : 8));
+  const size_t lds = 4 * (size_t)n * nc * sizeof(T);
+  if (lds > kLdsMax) return hipErrorInvalidValue;
+  auto launch = [&](auto kern) -> hipError_t {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const int nwg = (batch + 3) / 4;
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), lds, s, batch, n, nrhs, LU, perm, qbs, B);
+    return hipGetLastError();
+  };
+  if (nc == 1) return launch(&k_getrs_wave<T, 1>);
+  if (nc == 2) return launch(&k_getrs_wave<T, 2>);
+  if (nc == 4) return launch(&k_getrs_wave<T, 4>);
+  return launch(&k_getrs_wave<T, 8>);
+}
+
+hipError_t piv_to_perm_batched(int batch, int n, const int* ipiv, int64_t pbs, int* perm, int64_t qbs,
+                               hipStream_t s) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  const size_t lds = 2 * sizeof(int) * (size_t)n;
+  if (lds > kLdsMax) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_piv_to_perm),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_piv_to_perm, dim3(batch < 65535 ? batch : 65535), dim3(256), lds, s, batch, n, ipiv, pbs,
+                     perm, qbs);
+  return hipGetLastError();
+}
+
+template hipError_t getrs_perm<double>(int, int, int, View<double>, const int*, int64_t, View<double>,
+                                       hipStream_t);
+template hipError_t getrs_perm<float>(int, int, int, View<float>, const int*, int64_t, View<float>, hipStream_t);
 template hipError_t getrs_small<double>(int, int, int, View<double>, const int*, int64_t, View<double>,
                                         hipStream_t);
 template hipError_t getrs_small<float>(int, int, int, View<float>, const int*, int64_t, View<float>, hipStream_t);

@@ -69,6 +69,13 @@ hipError_t copy_batched(int batch, int rows, int cols, View<T> src, View<T> dst,
 template <typename T>
 hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
 // getrs for nrhs <= 8, non-transposed, one workgroup per matrix
+// LU solve with the composed pivot permutation (perm from piv_to_perm_batched),
+// nrhs <= 8, one wavefront per matrix
+template <typename T>
+hipError_t getrs_perm(int batch, int n, int nrhs, View<T> LU, const int* perm, int64_t qbs, View<T> B,
+                      hipStream_t s);
+hipError_t piv_to_perm_batched(int batch, int n, const int* ipiv, int64_t pbs, int* perm, int64_t qbs,
+                               hipStream_t s);
 template <typename T>
 hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
                        hipStream_t s);

@@ -35,6 +35,8 @@ def _load():
         "slab_laswp": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _vp], _int),
         "slab_getrf": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _vp], _int),
         "slab_getrs": ([_int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
+        "slab_piv_to_perm": ([_int, _int, _vp, _i64, _vp, _i64, _vp], _int),
+        "slab_getrs_perm": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_btri_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                               _i64, _vp, _vp], _int),
         "slab_btri_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
@@ -165,9 +167,28 @@ def getrf(A, ipiv, nelim=None, npiv=None, info=None):
     return A
 
 
-def getrs(LU, ipiv, B, trans=False):
+def piv_to_perm(ipiv, n=None):
+    """(batch, >= n) 1-based getrf pivots -> (batch, n) int32 row permutation:
+    row r of P B is row perm[r] of B."""
+    b = ipiv.shape[0]
+    n = ipiv.shape[1] if n is None else n
+    _require(ipiv.dtype == torch.int32 and ipiv.stride(-1) == 1, "ipiv layout")
+    perm = torch.empty(b, n, dtype=torch.int32, device=ipiv.device)
+    _check(LIB.slab_piv_to_perm(b, n, _p(ipiv), ipiv.stride(0), _p(perm), perm.stride(0), _stream()),
+           "slab_piv_to_perm")
+    return perm
+
+
+def getrs(LU, ipiv, B, trans=False, perm=None):
+    """In-place batched solve op(A) X = B with getrf factors; with perm (from
+    piv_to_perm) and at most 8 columns the one-wavefront-per-matrix kernel
+    runs."""
     _require(LU.dim() == 3 and B.dim() == 3 and LU.stride(2) == 1 and B.stride(2) == 1, "getrs layouts")
     b, n, _ = LU.shape
+    if perm is not None and not trans and 1 <= B.shape[2] <= 8:
+        _check(LIB.slab_getrs_perm(_dt(B), b, n, B.shape[2], _p(LU), LU.stride(1), LU.stride(0), _p(perm),
+                                   perm.stride(0), _p(B), B.stride(1), B.stride(0), _stream()), "slab_getrs_perm")
+        return B
     _check(LIB.slab_getrs(_dt(B), b, n, B.shape[2], int(trans), _p(LU), LU.stride(1), LU.stride(0), _p(ipiv),
                           ipiv.stride(0), _p(B), B.stride(1), B.stride(0), _stream()), "slab_getrs")
     return B

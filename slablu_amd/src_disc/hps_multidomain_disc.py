@@ -118,7 +118,7 @@ class HPS_Multidomain:
         DtN, M, ipiv, info = self.ops.factor(self.coefs, self.nboxes, dtype)
         if ev is not None:
             ev[1].record()
-        self.leaf = dict(DtN=DtN, M=M, ipiv=ipiv, info=info, dtype=dtype)
+        self.leaf = dict(DtN=DtN, M=M, ipiv=ipiv, perm=hip.piv_to_perm(ipiv), info=info, dtype=dtype)
         return DtN
 
     def get_DtNs(self, device=None, mode="build", data=0, ff_body_func=None):
@@ -145,7 +145,7 @@ class HPS_Multidomain:
         L = self.leaf
         u_bnd = u_bnd.to(L["dtype"]).contiguous()
         fb = None if ff_body_func is None else self.body_load(ff_body_func, u_bnd.shape[-1], L["dtype"])
-        uc = self.ops.interior(L["M"], L["ipiv"], u_bnd, fb)
+        uc = self.ops.interior(L["M"], L["ipiv"], u_bnd, fb, perm=L.get("perm"))
         uu = self.ops.assemble_nodes(u_bnd, uc)
         Au = self.ops.apply_aloc(uu, self.coefs)
         res = torch.zeros_like(uu)

@@ -77,8 +77,9 @@ class LeafOps:
         return DtN, M, ipiv, info
 
     # -- solve ---------------------------------------------------------------
-    def interior(self, M, ipiv, u_bnd, f_body=None):
-        """Leaf interiors u_c = A_cc^{-1}(f_c - A_cx u_x) (nboxes, ni, nrhs)."""
+    def interior(self, M, ipiv, u_bnd, f_body=None, perm=None):
+        """Leaf interiors u_c = A_cc^{-1}(f_c - A_cx u_x) (nboxes, ni, nrhs);
+        perm: the composed pivot permutation (hip.piv_to_perm) if kept."""
         ni = self.ni
         nb, _, nrhs = u_bnd.shape
         if f_body is None:
@@ -86,7 +87,7 @@ class LeafOps:
             hip.gemm(M[:, :, ni:], u_bnd, uc, alpha=-1.0, beta=0.0)
             return uc
         uc = f_body.to(M.dtype).contiguous().clone()
-        hip.getrs(M[:, :, :ni], ipiv, uc)
+        hip.getrs(M[:, :, :ni], ipiv, uc, perm=perm)
         hip.gemm(M[:, :, ni:], u_bnd, uc, alpha=-1.0, beta=1.0)
         return uc
 

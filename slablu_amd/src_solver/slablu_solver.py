@@ -473,7 +473,7 @@ class Domain_Solver:
         ops, lf = self.hps.ops, self.hps.leaf
         ni = ops.ni
         w = fb.to(lf["dtype"]).contiguous().clone()
-        hip.getrs(lf["M"][:, :, :ni], lf["ipiv"], w)
+        hip.getrs(lf["M"][:, :, :ni], lf["ipiv"], w, perm=lf.get("perm"))
         z = torch.empty(fb.shape[0], ops.nx, fb.shape[-1], dtype=w.dtype, device=w.device)
         hip.gemm(ops.typed(w.dtype)["Nx_c"], w, z, alpha=-1.0)
         y = torch.zeros(self.hps.nface, fb.shape[-1], dtype=w.dtype, device=w.device)
@@ -538,7 +538,7 @@ class Domain_Solver:
             dF = torch.zeros_like(x_face)
             dF[Ic] = dC.double()
             db = hps.faces_to_boxes(dF).to(self.wdt).contiguous()
-            dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt))
+            dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt), perm=lf.get("perm"))
             x_face += dF
             uu[:, ops.Jc] += dI.double()
             ub = hps.faces_to_boxes(x_face)
