@@ -9,13 +9,17 @@ on a 0.04 lattice with one row and one column removed (L-shaped waveguide),
 Dirichlet data from the free-space Green's function.  A step is one full
 factor + solve of the mixed-precision path (FP32 leaf/slab stages, FP64
 interface LU, FP64-residual refinement of the full discrete system until a
-correction is below 1e-9 of the solution).  The FP64 path is timed on the same problem and
-its solution is the reference for the error of the mixed solution.
+correction is below 1e-9 of the solution).  The FP64 path is timed on the
+same problem and its solution is the reference for the error of the mixed
+solution.
 
-Prints one JSON line (metric, value = DOF/s, ...), with the roofline of the
-dominant stage (leaf condensation) measured with HIP events on the stream
-the kernels run on, and a CPU baseline: the oracle on a bounded sample of
-the same workload on the host cores.
+Prints one JSON line (metric, value = DOF/s, ...).  roofline: the kernel
+with the largest share of the mixed step, the leaf interior solve
+(k_getrs_wave, HBM-bound: it streams every leaf's LU factors once per
+call), timed with HIP events on the stream it runs on against its
+algorithmic bytes; traffic is the measured HBM bytes per launch recorded in
+profiles/leaf_solve_traffic.json when present.  cpu_baseline: the oracle on
+a bounded sample of the same workload on the host cores.
 """
 import argparse
 import json
@@ -28,14 +32,49 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "factor+solve DOF/s at N≈1M p=22: mixed+refined vs FP64 at 1e-8 rel err"
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_HBM_GBS = 8000.0
 
 
-def leaf_flops(p):
-    """Algorithmic FLOPs of one leaf condensation: LU of the (p-2)^2 interior
-    block, its solve against the 4(p-2) face columns and the DtN product."""
-    ni, nb = (p - 2) ** 2, 4 * (p - 2)
-    return (2.0 / 3.0) * ni ** 3 + 2.0 * ni ** 2 * nb + 2.0 * nb ** 2 * (ni + nb)
+def leaf_solve_bytes(nboxes, ni, itemsize, nrhs=1):
+    """Algorithmic HBM bytes of one leaf interior solve launch: every leaf's
+    LU factors (ni x ni) once, the permutation, and the right-hand side read
+    and written."""
+    return nboxes * (ni * ni * itemsize + ni * 4 + 2 * ni * nrhs * itemsize)
+
+
+def time_leaf_solve(dom, reps=20):
+    """Average duration (ms) of one leaf interior solve launch on the factored
+    leaves of dom, with HIP events on the current stream."""
+    import torch
+    from slablu_amd import hip
+    lf = dom.hps.leaf
+    M = lf["M"]
+    ni = M.shape[1]
+    LU = M[:, :, :ni]
+    x0 = torch.randn(M.shape[0], ni, 1, dtype=M.dtype, device=M.device)
+    x = x0.clone()
+    hip.getrs(LU, lf["ipiv"], x, perm=lf["perm"])
+    ms = []
+    for _ in range(reps):
+        x.copy_(x0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        hip.getrs(LU, lf["ipiv"], x, perm=lf["perm"])
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return sum(ms) / len(ms), M.shape[0], ni, M.element_size()
+
+
+def measured_traffic():
+    """HBM bytes per leaf-solve launch from profiles/leaf_solve_traffic.json
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "leaf_solve_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
 
 
 def make_problem(n, p, ppw, precision):
@@ -152,6 +191,7 @@ def main():
     ts, tf, tsol, leaf_ms, sol_mixed, stages = run_path(dom, uu_dir, args.steps, args.warmup, leaf_events=True)
     refine_hist = list(dom.refine_history)
     nboxes = dom.hps.nboxes
+    roof_dom = time_leaf_solve(dom)
     del dom
     torch.cuda.empty_cache()
     fp64 = None
@@ -167,13 +207,15 @@ def main():
         del dom64
         torch.cuda.empty_cache()
     t_step = sum(ts) / len(ts)
-    leaf_avg = sum(leaf_ms) / len(leaf_ms)
-    flops = nboxes * leaf_flops(p)
-    ach = flops / (leaf_avg * 1e-3) / 1e12
-    roof = dict(bound="mfma", kernel="leaf condensation (FP32): slab_leaf_assemble + slab_getrf + slab_trsm "
-                                     "+ slab_gemm over %d leaves" % nboxes,
-                achieved=ach, peak=PEAK_TFLOPS["f32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["f32"],
-                traffic=None, algorithmic_flops=flops, avg_ms=leaf_avg)
+    roof = None
+    if roof_dom is not None:
+        avg_ms, nb, ni, isz = roof_dom
+        nbytes = leaf_solve_bytes(nb, ni, isz)
+        ach = nbytes / (avg_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", kernel="k_getrs_wave<float,1>: leaf interior solve A_cc^{-1} r over %d leaves "
+                                          "(%d x %d FP32 LU factors), the largest share of the mixed step" % (nb, ni, ni),
+                    achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                    traffic=measured_traffic(), algorithmic_bytes=nbytes, avg_ms=avg_ms)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
