@@ -173,6 +173,24 @@ int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D
                        const double* c1, const double* c2, const double* c,
                        const double* Nxtot, void* M, int64_t ldm, int64_t m_bs, void* stream);
 
+/* FP64 residual of the un-condensed leaf collocation system for every leaf
+ * in one launch (matrix-free): node values from interior values uc
+ * (nboxes, (p-2)^2, nrhs) at the Jc nodes and per-leaf face values ub
+ * (nboxes, 4(p-2), nrhs) through Intmap ((p^2-(p-2)^2) x 4(p-2), rows at the
+ * Jxreo nodes); outputs rI = -(A_loc u) at the interior nodes and
+ * fl = N_x u at the face nodes (N_x as nx_idx/nx_val, p entries per row);
+ * uu (nboxes, p^2, nrhs) receives all node values when not NULL.
+ * Coefficients as in slab_leaf_assemble.  Element (b, row, r) of every array
+ * sits at base + b*bs + row*ld + r.  Replaces the leaf residual
+ * Aloc @ u - f and the flux N_x u of src_disc/hps_parallel_leaf_ops.py:73-92
+ * and hps_multidomain_disc.py:227-243 (resloc), applied to a given iterate. */
+int slab_leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
+                       const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
+                       const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,
+                       const double* c12, const double* c1, const double* c2, const double* c, const int* nx_idx,
+                       const double* nx_val, double* rI, int64_t rI_ld, int64_t rI_bs, double* fl, int64_t fl_ld,
+                       int64_t fl_bs, double* uu, int64_t uu_ld, int64_t uu_bs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

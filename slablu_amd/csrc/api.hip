@@ -292,4 +292,23 @@ int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D
   return -1;
 }
 
+int slab_leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
+                       const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
+                       const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,
+                       const double* c12, const double* c1, const double* c2, const double* c, const int* nx_idx,
+                       const double* nx_val, double* rI, int64_t rI_ld, int64_t rI_bs, double* fl, int64_t fl_ld,
+                       int64_t fl_bs, double* uu, int64_t uu_ld, int64_t uu_bs, void* stream) {
+  if (nboxes < 0 || nrhs < 0 || p < 3 || p > 64) return -1;
+  if (!uc || !ub || !Intmap || !Jc || !Jxreo || !D || !Dsq || !c11 || !c22 || !nx_idx || !nx_val || !rI || !fl)
+    return -1;
+  const int f = p - 2;
+  if (uc_ld < nrhs || ub_ld < nrhs || rI_ld < nrhs || fl_ld < nrhs || (uu && uu_ld < nrhs)) return -1;
+  if (uc_bs < (int64_t)f * f * uc_ld || ub_bs < (int64_t)4 * f * ub_ld || rI_bs < (int64_t)f * f * rI_ld ||
+      fl_bs < (int64_t)4 * f * fl_ld || (uu && uu_bs < (int64_t)p * p * uu_ld))
+    return -1;
+  return status(leaf_residual(nboxes, p, nrhs, uc, uc_ld, uc_bs, ub, ub_ld, ub_bs, Intmap, Jc, Jxreo, D, Dsq, c11,
+                              c22, c12, c1, c2, c, nx_idx, nx_val, rI, rI_ld, rI_bs, fl, fl_ld, fl_bs, uu, uu_ld,
+                              uu_bs, S(stream)));
+}
+
 }  // extern "C"

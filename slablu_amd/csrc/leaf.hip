@@ -104,4 +104,129 @@ template hipError_t leaf_assemble<float>(int, int, int, const double*, const dou
                                          const double*, const double*, const double*, const double*, const double*,
                                          float*, int64_t, int64_t, hipStream_t);
 
+// FP64 residual of the un-condensed collocation system, matrix-free, one
+// workgroup per leaf.  The leaf's p^2 node values are assembled in LDS from
+// the interior values uc (at the Jc nodes) and the face values ub through
+// the face-to-node interpolation Intmap (rows at the Jxreo nodes, as
+// src_disc/hps_parallel_leaf_ops.py:84 assembles them); then
+//   rI = -(A_loc u) at the interior nodes (same operator as k_leaf_assemble)
+//   fl = N_x u, the outward normal derivatives at the 4(p-2) face nodes,
+// with N_x in compressed rows of p entries (nx_idx, nx_val).  uu, when not
+// NULL, receives all node values.  Element (b, row, r) of every array
+// argument sits at base + b*bs + row*ld + r.
+__global__ __launch_bounds__(256) void k_leaf_residual(
+    int nboxes, int p, int nrhs, const double* __restrict__ uc, int64_t uc_ld, int64_t uc_bs,
+    const double* __restrict__ ub, int64_t ub_ld, int64_t ub_bs, const double* __restrict__ Intmap,
+    const int* __restrict__ Jc, const int* __restrict__ Jxreo, const double* __restrict__ D,
+    const double* __restrict__ Dsq, const double* __restrict__ c11, const double* __restrict__ c22,
+    const double* __restrict__ c12, const double* __restrict__ c1, const double* __restrict__ c2,
+    const double* __restrict__ c, const int* __restrict__ nx_idx, const double* __restrict__ nx_val,
+    double* __restrict__ rI, int64_t rI_ld, int64_t rI_bs, double* __restrict__ fl, int64_t fl_ld, int64_t fl_bs,
+    double* __restrict__ uu, int64_t uu_ld, int64_t uu_bs) {
+  extern __shared__ double smem[];
+  const int f = p - 2, ni = f * f, nx = 4 * f, p2 = p * p, nxn = p2 - ni;
+  double* s_D = smem;             // p*p
+  double* s_Dsq = s_D + p2;       // p*p
+  double* s_u = s_Dsq + p2;       // p*p node values
+  double* s_ux = s_u + p2;        // p*p x-derivatives (c12 term)
+  double* s_ub = s_ux + p2;       // nx face values
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < p2; t += blockDim.x) {
+    s_D[t] = D[t];
+    s_Dsq[t] = Dsq[t];
+  }
+  const int64_t cb = (int64_t)b * p2;
+  for (int r = 0; r < nrhs; ++r) {
+    __syncthreads();
+    for (int t = tid; t < nx; t += blockDim.x) s_ub[t] = ub[(int64_t)b * ub_bs + (int64_t)t * ub_ld + r];
+    __syncthreads();
+    for (int t = tid; t < p2; t += blockDim.x) {
+      int node;
+      double v;
+      if (t < ni) {
+        node = Jc[t];
+        v = uc[(int64_t)b * uc_bs + (int64_t)t * uc_ld + r];
+      } else {
+        const int q = t - ni;
+        node = Jxreo[q];
+        const double* row = Intmap + (int64_t)q * nx;
+        double a0 = 0.0, a1 = 0.0;
+        int k = 0;
+        for (; k + 1 < nx; k += 2) {
+          a0 += row[k] * s_ub[k];
+          a1 += row[k + 1] * s_ub[k + 1];
+        }
+        if (k < nx) a0 += row[k] * s_ub[k];
+        v = a0 + a1;
+      }
+      s_u[node] = v;
+      if (uu) uu[(int64_t)b * uu_bs + (int64_t)node * uu_ld + r] = v;
+    }
+    __syncthreads();
+    if (c12) {
+      for (int t = tid; t < p2; t += blockDim.x) {
+        const int i1 = t / p, i2 = t % p;
+        double a = 0.0;
+        for (int j = 0; j < p; ++j) a += s_D[i1 * p + j] * s_u[j * p + i2];
+        s_ux[t] = a;
+      }
+      __syncthreads();
+    }
+    (void)nxn;
+    for (int t = tid; t < ni + nx; t += blockDim.x) {
+      if (t < ni) {
+        const int i1 = 1 + t / f, i2 = 1 + t % f;
+        const int node = i1 * p + i2;
+        double axx = 0.0, ayy = 0.0, ax = 0.0, ay = 0.0;
+        for (int j = 0; j < p; ++j) {
+          const double ux = s_u[j * p + i2], uy = s_u[i1 * p + j];
+          axx += s_Dsq[i1 * p + j] * ux;
+          ayy += s_Dsq[i2 * p + j] * uy;
+          ax += s_D[i1 * p + j] * ux;
+          ay += s_D[i2 * p + j] * uy;
+        }
+        double res = -c11[cb + node] * axx;
+        res -= c22[cb + node] * ayy;
+        if (c12) {
+          double axy = 0.0;
+          for (int j = 0; j < p; ++j) axy += s_D[i2 * p + j] * s_ux[i1 * p + j];
+          res -= 2.0 * c12[cb + node] * axy;
+        }
+        if (c1) res -= c1[cb + node] * ax;
+        if (c2) res -= c2[cb + node] * ay;
+        if (c) res += c[cb + node] * s_u[node];
+        rI[(int64_t)b * rI_bs + (int64_t)t * rI_ld + r] = -res;
+      } else {
+        const int q = t - ni;
+        const int* ix = nx_idx + q * p;
+        const double* vx = nx_val + q * p;
+        double a0 = 0.0, a1 = 0.0;
+        int k = 0;
+        for (; k + 1 < p; k += 2) {
+          a0 += vx[k] * s_u[ix[k]];
+          a1 += vx[k + 1] * s_u[ix[k + 1]];
+        }
+        if (k < p) a0 += vx[k] * s_u[ix[k]];
+        fl[(int64_t)b * fl_bs + (int64_t)q * fl_ld + r] = a0 + a1;
+      }
+    }
+  }
+}
+
+hipError_t leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
+                         const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
+                         const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,
+                         const double* c12, const double* c1, const double* c2, const double* c, const int* nx_idx,
+                         const double* nx_val, double* rI, int64_t rI_ld, int64_t rI_bs, double* fl, int64_t fl_ld,
+                         int64_t fl_bs, double* uu, int64_t uu_ld, int64_t uu_bs, hipStream_t s) {
+  if (nboxes == 0 || nrhs == 0) return hipSuccess;
+  const int f = p - 2;
+  const size_t lds = sizeof(double) * (4 * (size_t)p * p + 4 * f);
+  hipLaunchKernelGGL(k_leaf_residual, dim3(nboxes), dim3(256), lds, s, nboxes, p, nrhs, uc, uc_ld, uc_bs, ub, ub_ld,
+                     ub_bs, Intmap, Jc, Jxreo, D, Dsq, c11, c22, c12, c1, c2, c, nx_idx, nx_val, rI, rI_ld, rI_bs, fl,
+                     fl_ld, fl_bs, uu, uu_ld, uu_bs);
+  return hipGetLastError();
+}
+
 }  // namespace slab

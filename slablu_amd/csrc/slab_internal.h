@@ -55,6 +55,12 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
 }  // namespace slab
 
 namespace slab {
+hipError_t leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
+                         const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
+                         const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,
+                         const double* c12, const double* c1, const double* c2, const double* c, const int* nx_idx,
+                         const double* nx_val, double* rI, int64_t rI_ld, int64_t rI_bs, double* fl, int64_t fl_ld,
+                         int64_t fl_bs, double* uu, int64_t uu_ld, int64_t uu_bs, hipStream_t s);
 template <typename T>
 hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const double* Dsq, const double* c11,
                          const double* c22, const double* c12, const double* c1, const double* c2, const double* c,

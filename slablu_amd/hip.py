@@ -53,6 +53,9 @@ def _load():
                                  _i64, _i64, _int, _vp, _vp], _int),
         "slab_leaf_assemble": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                 _vp], _int),
+        "slab_leaf_residual": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp],
+                               _int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -287,3 +290,34 @@ def leaf_assemble(M, p, border, D, Dsq, coefs, Nxtot):
     _check(LIB.slab_leaf_assemble(_dt(M), M.shape[0], p, int(border), _p(D), _p(Dsq), *[_p(a) for a in args],
                                   _p(Nxtot), _p(M), M.stride(1), M.stride(0), _stream()), "slab_leaf_assemble")
     return M
+
+
+def leaf_residual(uc, ub, tables, coefs, p, uu=None):
+    """FP64 leaf residuals of the un-condensed system for all leaves: uc
+    (nboxes, (p-2)^2, nrhs) interior values, ub (nboxes, 4(p-2), nrhs) face
+    values -> (rI = -(A_loc u) at the interior nodes, fl = N_x u); uu
+    (nboxes, p^2, nrhs), if given, receives all node values.  tables: dict
+    with Intmap, Jc, Jxreo, D, Dsq, nx_idx, nx_val (device)."""
+    nb, ni, nrhs = uc.shape
+    f = p - 2
+    _require(uc.dtype == torch.float64 and ub.dtype == torch.float64, "leaf_residual works in FP64")
+    _require(ni == f * f and ub.shape == (nb, 4 * f, nrhs), "leaf_residual shapes")
+    _require(uc.stride(2) == 1 and ub.stride(2) == 1, "leaf_residual layouts")
+    for k in ("c11", "c22"):
+        _require(coefs.get(k) is not None, "coefficient %s required" % k)
+    args = [coefs.get(k) for k in ("c11", "c22", "c12", "c1", "c2", "c")]
+    for a in args:
+        _require(a is None or (a.dtype == torch.float64 and a.is_contiguous() and a.numel() == nb * p * p),
+                 "coefficient arrays must be contiguous FP64 of nboxes*p*p")
+    rI = torch.empty(nb, ni, nrhs, dtype=torch.float64, device=uc.device)
+    fl = torch.empty(nb, 4 * f, nrhs, dtype=torch.float64, device=uc.device)
+    if uu is not None:
+        _require(uu.shape == (nb, p * p, nrhs) and uu.stride(2) == 1 and uu.dtype == torch.float64, "uu layout")
+    t = tables
+    _check(LIB.slab_leaf_residual(nb, p, nrhs, _p(uc), uc.stride(1), uc.stride(0), _p(ub), ub.stride(1),
+                                  ub.stride(0), _p(t["Intmap"]), _p(t["Jc"]), _p(t["Jxreo"]), _p(t["D"]),
+                                  _p(t["Dsq"]), *[_p(a) for a in args], _p(t["nx_idx"]), _p(t["nx_val"]), _p(rI),
+                                  rI.stride(1), rI.stride(0), _p(fl), fl.stride(1), fl.stride(0), _p(uu),
+                                  uu.stride(1) if uu is not None else 0, uu.stride(0) if uu is not None else 0,
+                                  _stream()), "slab_leaf_residual")
+    return rI, fl

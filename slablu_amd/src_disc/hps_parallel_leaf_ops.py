@@ -44,6 +44,20 @@ class LeafOps:
         self.Jx = t(JJ.Jx, torch.long)
         self.Jxreo = t(JJ.Jxreorder, torch.long)
         self.Intmap = t(H.Interp_mat)
+        # N_x in compressed rows of p entries (each row is a derivative along
+        # one grid line), for the fused residual kernel
+        Nx = np.asarray(H.Nx, dtype=np.float64)
+        nx_idx = np.zeros((Nx.shape[0], p), dtype=np.int32)
+        nx_val = np.zeros((Nx.shape[0], p), dtype=np.float64)
+        for r in range(Nx.shape[0]):
+            nz = np.nonzero(Nx[r])[0]
+            if len(nz) > p:
+                raise ValueError("N_x row %d has %d entries (at most p=%d expected)" % (r, len(nz), p))
+            nx_idx[r, :len(nz)] = nz
+            nx_val[r, :len(nz)] = Nx[r, nz]
+        self.res_tables = dict(Intmap=self.Intmap.contiguous(), Jc=t(JJ.Jc, torch.int32),
+                               Jxreo=t(JJ.Jxreorder, torch.int32), D=self.D.contiguous(),
+                               Dsq=self.Dsq.contiguous(), nx_idx=t(nx_idx, torch.int32), nx_val=t(nx_val))
         self._typed = {}
 
     def typed(self, dtype):
@@ -141,6 +155,13 @@ class LeafOps:
         if coefs.get("c") is not None:
             res = res + cf("c") * uu
         return res
+
+    def residual(self, uc, ub, coefs, uu=None):
+        """FP64 residual pieces at an iterate given by interior values uc
+        (nb, ni, nrhs) and per-leaf face values ub (nb, nx, nrhs), in one
+        launch: (rI = -(A_loc u) at the interior nodes, N_x u at the face
+        nodes); uu, if given, receives all node values."""
+        return hip.leaf_residual(uc.contiguous(), ub.contiguous(), self.res_tables, coefs, self.p, uu=uu)
 
     def flux(self, uu):
         """Outward normal derivatives Nx u on the 4(p-2) face nodes of every

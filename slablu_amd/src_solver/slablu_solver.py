@@ -24,8 +24,11 @@ sweep, slab sweeps, leaf recovery (slablu_solver.py:489-666).
 
 precision='mixed' runs steps 1-4a in FP32 (interface LU in FP64) and
 refines the full discrete system (leaf collocation equations + flux
-continuity on the faces) with FP64 residuals until the correction falls
-below `refine_tol`.
+continuity on the faces) with FP64 residuals until the change of the face
+solution falls below `refine_tol` (relative).  The default refinement is
+flexible GMRES (FP64 Arnoldi on the un-condensed operator, right-
+preconditioned by the FP32 two-level solve); refine_method='richardson' runs
+plain iterative refinement instead (one FP32 solve per correction).
 """
 import time
 from functools import reduce
@@ -66,7 +69,7 @@ def _blockdiag(blocks):
 
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
-                 device=None, refine_tol=1e-9, refine_maxit=12):
+                 device=None, refine_tol=1e-9, refine_maxit=12, refine_method="fgmres", refine_restart=10):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
         if periodic_bc:
@@ -77,7 +80,10 @@ class Domain_Solver:
         self.periodic_bc = periodic_bc
         self.buf_constant = buf_constant
         self.precision = precision
+        if refine_method not in ("fgmres", "richardson"):
+            raise ValueError("refine_method must be 'fgmres' or 'richardson'")
         self.refine_tol, self.refine_maxit = refine_tol, refine_maxit
+        self.refine_method, self.refine_restart = refine_method, refine_restart
         # synchronize inside build to split stage timings (bench/profiling only)
         self.stage_sync = True
         self.device = torch.device(device) if device is not None else torch.device("cuda")
@@ -94,6 +100,12 @@ class Domain_Solver:
         I_Ddir = self.inds_pans[:, self.I_D].flatten()
         I_Udir = self.inds_pans[:, self.I_U].flatten()
         self.I_Xtot = torch.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+
+    def _Ic(self):
+        """I_Ctot on the device (cached: no host-to-device copy per call)."""
+        if getattr(self, "_Ic_dev", None) is None:
+            self._Ic_dev = self.I_Ctot.to(self.device)
+        return self._Ic_dev
 
     # ------------------------------------------------------------------ setup
     def hps_disc(self, box_geom, a, p, pdo_op):
@@ -440,14 +452,14 @@ class Domain_Solver:
         return x, uu_dir
 
     def _face_residual(self, x_face, DtN):
-        Ic = self.I_Ctot.to(self.device)
+        Ic = self._Ic()
         return self.hps.apply_A(x_face, DtN)[Ic]
 
     def solve_helper(self, uu_dir_func, ff_body_func=None):
         """Two-level solve; returns (face solution with Dirichlet values,
         relative residual of the reduced system, time)."""
         dev = self.device
-        Ic = self.I_Ctot.to(dev)
+        Ic = self._Ic()
         torch.cuda.synchronize()
         tic = time.time()
         x_face, uu_dir = self.get_rhs(uu_dir_func)
@@ -501,57 +513,164 @@ class Domain_Solver:
         return uu.double().cpu(), rel, true_err, resloc, toc
 
     # ------------------------------------------------------- mixed precision
-    def full_residual(self, x_face, uu):
-        """FP64 residual of the un-condensed system at the current iterate:
-        r_I = f - A_loc u at leaf interiors, r_F = -(sum of outward fluxes) at
-        the non-Dirichlet face DOFs."""
+    def full_residual(self, x_face, uc, uu=None):
+        """FP64 residual of the un-condensed system at the iterate given by
+        the face vector x_face (Dirichlet values included) and the leaf
+        interior values uc (nboxes, ni, nc): r_I = f - A_loc u at the leaf
+        interiors, r_F = -(sum of outward fluxes) at the face DOFs I_Ctot.
+        One fused leaf launch; uu, if given, receives all node values."""
         hps = self.hps
-        ops = hps.ops
-        Au = ops.apply_aloc(uu, hps.coefs)
-        rI = -Au[:, ops.Jc]
-        fl = ops.flux(uu)
-        y = torch.zeros(hps.nface, uu.shape[-1], dtype=torch.float64, device=self.device)
-        y.index_add_(0, hps.Iext_d.reshape(-1), fl.reshape(-1, uu.shape[-1]))
-        return rI, -y[self.I_Ctot.to(self.device)]
+        nc = x_face.shape[-1]
+        rI, fl = hps.ops.residual(uc, hps.faces_to_boxes(x_face), hps.coefs, uu=uu)
+        y = torch.zeros(hps.nface, nc, dtype=torch.float64, device=self.device)
+        y.index_add_(0, hps.Iext_d.reshape(-1), fl.reshape(-1, nc))
+        return rI, -y[self._Ic()]
+
+    def _correction(self, rI, rF):
+        """FP32 two-level solve of A d = r for a residual (rI, rF) of the full
+        system: the face correction from the condensed system (interior
+        residuals folded in through the leaf solves), then the leaf interiors.
+        Returns (dI (nboxes, ni, nc), dC (|I_Ctot|, nc)) in FP64."""
+        hps, ops, lf = self.hps, self.hps.ops, self.hps.leaf
+        Ic = self._Ic()
+        ff = rF + self._body_reduce(rI)[Ic].double()
+        dC = self.solve_reduced(ff.to(self.wdt)).double()
+        dF = torch.zeros(hps.nface, dC.shape[-1], dtype=torch.float64, device=self.device)
+        dF[Ic] = dC
+        db = hps.faces_to_boxes(dF).to(self.wdt).contiguous()
+        dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt).contiguous(), perm=lf.get("perm"))
+        return dI.double(), dC
+
+    def _apply_full(self, dI, dC):
+        """A d of the un-condensed system (FP64) for a vector with zero
+        Dirichlet data: interior values dI, face values dC on I_Ctot."""
+        x = torch.zeros(self.hps.nface, dC.shape[-1], dtype=torch.float64, device=self.device)
+        x[self._Ic()] = dC
+        rI, rF = self.full_residual(x, dI)
+        return -rI, -rF
+
+    def _refine_richardson(self, x_face, uc):
+        """x <- x + M^{-1}(b - A x) until the face correction is small."""
+        Ic = self._Ic()
+        for it in range(self.refine_maxit):
+            rI, rF = self.full_residual(x_face, uc)
+            dI, dC = self._correction(rI, rF)
+            x_face[Ic] += dC
+            uc += dI
+            rel_corr = (torch.linalg.norm(dC) / torch.linalg.norm(x_face)).item()
+            self.refine_history.append(rel_corr)
+            if rel_corr <= self.refine_tol:
+                break
+
+    def _refine_fgmres(self, x_face, uc):
+        """Flexible GMRES on the full system, right-preconditioned by the FP32
+        two-level solve M: the Arnoldi relation A Z_j = V_{j+1} H_j holds in
+        FP64 for the stored preconditioned directions Z_j = M^{-1} V_j, so the
+        FP64 residual ||b - A x|| is minimised over x0 + span(Z_j) even though
+        M is applied in FP32.  Stops when the change of the face solution
+        between consecutive iterates is below refine_tol relative to the face
+        solution (the quantity plain refinement tests), restarting every
+        refine_restart steps."""
+        hps, ops = self.hps, self.hps.ops
+        dev = self.device
+        Ic = self._Ic()
+        nb, ni = hps.nboxes, ops.ni
+        nI = nb * ni
+        nc = x_face.shape[-1]
+        ntot = nI + Ic.shape[0]
+        # Krylov storage (V and Z: 2 * kmax + 1 vectors per right-hand side)
+        kmax = max(2, min(self.refine_restart, (16 << 30) // max(1, 16 * ntot * nc)))
+
+        def pack(aI, aC, out):
+            out.view(nc, ntot)[:, :nI] = aI.reshape(nI, nc).T
+            out.view(nc, ntot)[:, nI:] = aC.T
+
+        def unpack(v):
+            # fresh row-major tensors (a (..., 1) view of v.T is "contiguous"
+            # with a non-unit last stride)
+            aI = torch.empty(nb, ni, nc, dtype=torch.float64, device=dev)
+            aC = torch.empty(ntot - nI, nc, dtype=torch.float64, device=dev)
+            aI.view(nI, nc).copy_(v[:, :nI].T)
+            aC.copy_(v[:, nI:].T)
+            return aI, aC
+
+        steps = 0
+        done = False
+        while steps < self.refine_maxit and not done:
+            rI, rF = self.full_residual(x_face, uc)
+            V = torch.empty(nc, kmax + 1, ntot, dtype=torch.float64, device=dev)
+            Z = torch.empty(nc, kmax, ntot, dtype=torch.float64, device=dev)
+            pack(rI, rF, V[:, 0])
+            beta = torch.linalg.norm(V[:, 0], dim=1)
+            V[:, 0] /= beta.clamp_min(1e-300)[:, None]
+            beta = beta.cpu().numpy()
+            H = np.zeros((nc, kmax + 1, kmax))
+            xc0 = x_face[Ic].T.contiguous()                       # (nc, |I_Ctot|)
+            fixed2 = (torch.linalg.norm(x_face) ** 2 - torch.linalg.norm(xc0) ** 2).clamp_min(0)
+            y_prev = np.zeros((nc, 0))
+            for j in range(kmax):
+                zI, zC = self._correction(*unpack(V[:, j]))
+                pack(zI, zC, Z[:, j])
+                wI, wF = self._apply_full(zI, zC)
+                w = torch.empty(nc, ntot, dtype=torch.float64, device=dev)
+                pack(wI, wF, w)
+                h = torch.zeros(nc, j + 2, dtype=torch.float64, device=dev)
+                for _ in range(2):                                  # classical Gram-Schmidt, twice
+                    c = (V[:, :j + 1] * w.unsqueeze(1)).sum(-1)     # (nc, j+1)
+                    for i in range(j + 1):
+                        w.addcmul_(V[:, i], c[:, i:i + 1], value=-1.0)
+                    h[:, :j + 1] += c
+                h[:, j + 1] = torch.linalg.norm(w, dim=1)
+                V[:, j + 1] = w / h[:, j + 1].clamp_min(1e-300)[:, None]
+                H[:, :j + 2, j] = h.cpu().numpy()
+                y = np.zeros((nc, j + 1))
+                for col in range(nc):
+                    rhs = np.zeros(j + 2)
+                    rhs[0] = beta[col]
+                    y[col] = np.linalg.lstsq(H[col, :j + 2, :j + 1], rhs, rcond=None)[0]
+                dy = y - np.pad(y_prev, ((0, 0), (0, 1)))
+                yt = torch.from_numpy(np.stack((y, dy), 1)).to(dev)           # (nc, 2, j+1)
+                comb = (yt.unsqueeze(-1) * Z[:, None, :j + 1, nI:]).sum(2)      # (nc, 2, |I_Ctot|)
+                xnorm = torch.sqrt(fixed2 + torch.linalg.norm(xc0 + comb[:, 0]) ** 2)
+                rel = (torch.linalg.norm(comb[:, 1]) / xnorm).item()
+                self.refine_history.append(rel)
+                y_prev = y
+                steps += 1
+                if rel <= self.refine_tol or steps >= self.refine_maxit:
+                    done = rel <= self.refine_tol
+                    break
+            yd = torch.from_numpy(y_prev).to(dev)
+            upd = torch.zeros(nc, ntot, dtype=torch.float64, device=dev)
+            for i in range(y_prev.shape[1]):
+                upd.addcmul_(Z[:, i], yd[:, i:i + 1])
+            dI, dC = unpack(upd)
+            x_face[Ic] += dC
+            uc += dI
+            del V, Z
 
     def solve_refined(self, uu_dir_func, ff_body_func=None):
         """Mixed precision: FP32 factors, FP64 residuals of the full system,
-        corrections until ||delta|| <= refine_tol ||u||."""
+        flexible GMRES (or plain refinement) until the face solution changes
+        by at most refine_tol (relative)."""
         if ff_body_func is not None:
             raise NotImplementedError("body loads with precision='mixed' are not provided")
         hps, ops = self.hps, self.hps.ops
         dev = self.device
-        Ic = self.I_Ctot.to(dev)
         lf = hps.leaf
         torch.cuda.synchronize()
         tic = time.time()
         x_face, uu_dir = self.get_rhs(uu_dir_func)
         nc = uu_dir.shape[1]
-        ub = hps.faces_to_boxes(x_face)
-        uu = ops.assemble_nodes(ub, torch.zeros(hps.nboxes, ops.ni, nc, dtype=torch.float64, device=dev))
+        uc = torch.zeros(hps.nboxes, ops.ni, nc, dtype=torch.float64, device=dev)
         self.refine_history = []
-        for it in range(self.refine_maxit):
-            rI, rF = self.full_residual(x_face, uu)
-            # correction: A dF = rF - sum_b P_b Nc Acc^{-1} rI ; dI = Acc^{-1} rI - S' dF
-            ff = rF + self._body_reduce(rI)[Ic].double()
-            dC = self.solve_reduced(ff.to(self.wdt))
-            dF = torch.zeros_like(x_face)
-            dF[Ic] = dC.double()
-            db = hps.faces_to_boxes(dF).to(self.wdt).contiguous()
-            dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt), perm=lf.get("perm"))
-            x_face += dF
-            uu[:, ops.Jc] += dI.double()
-            ub = hps.faces_to_boxes(x_face)
-            ureo = torch.empty(hps.nboxes, ops.Intmap.shape[0], nc, dtype=torch.float64, device=dev)
-            hip.gemm(ops.Intmap, ub, ureo)
-            uu[:, ops.Jxreo] = ureo
-            rel_corr = (torch.linalg.norm(dF) / torch.linalg.norm(x_face)).item()
-            self.refine_history.append(rel_corr)
-            if rel_corr <= self.refine_tol:
-                break
+        if self.refine_method == "fgmres":
+            self._refine_fgmres(x_face, uc)
+        else:
+            self._refine_richardson(x_face, uc)
+        uu = torch.empty(hps.nboxes, ops.p * ops.p, nc, dtype=torch.float64, device=dev)
+        rI, rF = self.full_residual(x_face, uc, uu=uu)
         torch.cuda.synchronize()
         toc = time.time() - tic
-        rI, rF = self.full_residual(x_face, uu)
         resloc = torch.linalg.norm(rI).item()
         ff0 = -self._face_residual(self.get_rhs(uu_dir_func)[0], lf["DtN"]).double()
         rel = (torch.linalg.norm(rF) / torch.linalg.norm(ff0)).item()

@@ -201,6 +201,54 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
     assert _rel(f, ref) < 200 * TOL[dtype]
 
 
+@pytest.mark.parametrize("p", [10, 22])
+@pytest.mark.parametrize("general", [False, True])
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_leaf_residual_matches_dense_operator(p, general, nrhs):
+    """Fused FP64 leaf residual (node assembly through Interp_mat, -(A_loc u)
+    at the interiors, N_x u at the faces) against the oracle's dense p^2 x p^2
+    leaf operator and N_x."""
+    from oracle import hps_oracle as O
+    from slablu_amd.src_disc.hps_parallel_leaf_ops import LeafOps
+    from slablu_amd.src_disc.hps_subdomain_disc import HPS_Disc
+    hip = _hip()
+    a = 0.013
+    ones = lambda xx: np.ones(xx.shape[0])
+    if general:
+        pdo = O.PDO(lambda xx: 1 + 0.3 * np.sin(7 * xx[:, 0]), lambda xx: 2 + xx[:, 1] ** 2,
+                    c12=lambda xx: 0.1 * xx[:, 0] * xx[:, 1], c1=lambda xx: np.cos(xx[:, 1]),
+                    c2=lambda xx: -xx[:, 0], c=lambda xx: -400 * (1 + 0.2 * np.cos(9 * xx[:, 0])))
+    else:
+        pdo = O.PDO(ones, ones, c=lambda xx: -300.0 * np.ones(xx.shape[0]))
+    hps = O.HPSOracle(pdo, np.array([[0.0, 6 * a], [0.0, 4 * a]]), a, p)
+    nb = hps.nboxes
+    A = hps.aloc(np.arange(nb))
+    Jc, Jxr = hps.JJ["Jc"], hps.JJ["Jxreorder"]
+    xx = hps.grid_xx.reshape(-1, 2)
+    coefs = {}
+    for name in ("c11", "c22", "c12", "c1", "c2", "c"):
+        fn = getattr(pdo, name)
+        if fn is not None:
+            coefs[name] = torch.tensor(np.asarray(fn(xx), dtype=np.float64).reshape(-1), device="cuda")
+    ops = LeafOps(HPS_Disc(a, p), "cuda")
+    f = p - 2
+    ni, nx = f * f, 4 * f
+    rng = np.random.default_rng(p + 10 * nrhs)
+    uc = rng.standard_normal((nb, ni, nrhs))
+    ub = rng.standard_normal((nb, nx, nrhs))
+    u = np.zeros((nb, p * p, nrhs))
+    u[:, Jc] = uc
+    u[:, Jxr] = hps.Interp_mat[None] @ ub
+    rI_ref = -(A @ u)[:, Jc]
+    fl_ref = hps.Nx[None] @ u
+    uu = torch.empty(nb, p * p, nrhs, dtype=torch.float64, device="cuda")
+    rI, fl = ops.residual(torch.tensor(uc, device="cuda"), torch.tensor(ub, device="cuda"), coefs, uu=uu)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(uu.cpu().numpy() - u) / np.linalg.norm(u) < 1e-14
+    assert np.linalg.norm(rI.cpu().numpy() - rI_ref) / np.linalg.norm(rI_ref) < 1e-13
+    assert np.linalg.norm(fl.cpu().numpy() - fl_ref) / np.linalg.norm(fl_ref) < 1e-13
+
+
 @pytest.mark.parametrize("p", [10, 16, 22])
 @pytest.mark.parametrize("general", [False, True])
 def test_leaf_assemble_bit_exact(p, general):
