@@ -482,103 +482,145 @@ __global__ __launch_bounds__(256) void k_laswp_panel(int batch, int ncols, int k
 
 // ---------------------------------------------------------------------------
 // LU panel in registers: rows k0..nrows-1 (R of them), columns k0..k0+w-1
-// (w <= W) of one matrix per workgroup; thread t holds panel rows t + 256 s
-// (s < RP) across all W columns.  Per column: argmax over the thread's
-// rows, wavefront argmax (DPP), merge of the four wavefronts through LDS;
-// the owners of the pivot row and of row j write them to LDS, the two rows
-// trade places and every thread divides its rows by the pivot and updates
-// them in registers.  Two barriers per
-// column.  Rows at or past npiv are updated but never chosen as pivots.
+// (w <= W) of one matrix per NT-thread workgroup; thread t holds panel rows
+// t + NT s (s < RP) across all W columns.
+//
+// Rows never move between registers.  Each thread keeps, for each of its
+// rows, the row's current logical position (where the interchanges so far
+// have sent it) and whether it has been a pivot row.  Per column j:
+//  * every thread picks its best not-yet-pivot row (largest |a|, ties to the
+//    smaller logical position, as LAPACK's i?amax), the wavefront reduces it
+//    with DPP, and the lane owning the wavefront's winner publishes the row,
+//    its value and its logical position in LDS;
+//  * after one barrier every thread merges the wavefront candidates into the
+//    pivot (logical position p) and reads the pivot row from LDS; the
+//    interchange j <-> p only relabels logical positions, the pivot row is
+//    marked, and every thread divides its remaining rows by the pivot and
+//    updates them in registers.
+// Candidate buffers alternate between two slots, so writes for column j+2
+// never meet reads of column j.  Rows at or past npiv are updated but never
+// chosen.  The panel moves between HBM and registers through a
+// per-wavefront LDS tile (row-contiguous global accesses); on the way out
+// each row goes to its final logical position.
 // ---------------------------------------------------------------------------
-template <typename T, int W, int RP>
-__global__ __launch_bounds__(256) void k_getf2_reg(int batch, int nrows, int k0, int w, int npiv, View<T> A,
-                                                   int* ipiv, int64_t pbs, int* info) {
-  __shared__ T prow[2][W];
-  __shared__ T jrow[2][W];
-  __shared__ T red_v[2][4];
-  __shared__ int red_i[2][4];
+template <typename T, int W, int RP, int NT>
+__global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, int w, int npiv, View<T> A,
+                                                  int* ipiv, int64_t pbs, int* info) {
+  constexpr int NW = NT / 64;
+  constexpr int TW = (sizeof(T) == 8 && W > 16) ? 16 : W;  // staging tile width
+  static_assert(W % TW == 0 && NT % 64 == 0 && NT <= 1024, "panel shape");
+  __shared__ T cand[2][NW][W];
+  __shared__ unsigned long long red_k[2][NW];
+  __shared__ T stage[NW][64][TW + 1];
+  __shared__ int stage_pos[NW][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int R = nrows - k0, Rp = npiv - k0;
   const int rm1 = R - 1, wm1 = w - 1;
   for (int b = blockIdx.x; b < batch; b += gridDim.x) {
     T x[RP][W];
+    int lp[RP];     // logical position of each row
+    bool dn[RP];    // pivot row already (or a padding row)
 #pragma unroll
     for (int s = 0; s < RP; ++s) {
-      const int rr = tid + 256 * s;
-      const T* rowp = A.at(b, k0 + (rr < rm1 ? rr : rm1), k0);
-#pragma unroll
-      for (int c = 0; c < W; ++c) {
-        const T v = rowp[(int64_t)(c < wm1 ? c : wm1) * A.cs];
-        x[s][c] = (rr < R && c < w) ? v : T(0);
-      }
+      lp[s] = tid + NT * s;
+      dn[s] = lp[s] >= R;
     }
-static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
+    // coalesced load: a wavefront moves its 64 rows of slot s in TW-wide
+    // column tiles, 64 / TW rows per load instruction
+    static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
+      constexpr int sl = decltype(sc)::value;
+      const int r0 = NT * sl + 64 * wave;
+      static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
+        constexpr int c0 = decltype(tc)::value * TW;
+#pragma unroll
+        for (int q = 0; q < TW; ++q) {
+          const int e = 64 * q + lane;
+          const int rr = r0 + e / TW, cc = c0 + e % TW;
+          stage[wave][e / TW][e % TW] = *A.at(b, k0 + (rr < rm1 ? rr : rm1), k0 + (cc < wm1 ? cc : wm1));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+          const T v = stage[wave][lane][c];
+          x[sl][c0 + c] = (!dn[sl] && c0 + c < w) ? v : T(0);
+        }
+        __syncthreads();
+      });
+    });
+    static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
       constexpr int j = decltype(jc)::value;
       if (j < w) {
         const int par = j & 1;
-        T best = T(-1);
-        int bi = INT_MAX;
+        unsigned long long best = 0;
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
-          const int rr = tid + 256 * s;
-          const T v = tabs(x[s][j]);
-          const bool ok = (rr >= j) & (rr < Rp) & (v > best);
-          best = ok ? v : best;
-          bi = ok ? rr : bi;
+          const T v = x[s][j];
+          const bool ok = !dn[s] & (lp[s] < Rp) & (v == v);
+          const unsigned long long k = ok ? pivot_key(v, lp[s]) : 0ull;
+          best = umax64(best, k);
         }
-        wave_argmax(best, bi);
-        if (lane == 0) {
-          red_v[par][wave] = best;
-          red_i[par][wave] = bi;
-        }
-        __syncthreads();
-        T bv = red_v[par][0];
-        int p = red_i[par][0];
+        best = wave_max_u64(best);
+        if (lane == 0) red_k[par][wave] = best;
+        {
+          const int bi = pivot_key_index(best, T(0));
+          bool own = false;
 #pragma unroll
-        for (int q = 1; q < 4; ++q) argmax_merge(bv, p, red_v[par][q], red_i[par][q]);
-        if (p < j || p >= Rp) p = j;  // no valid candidate (e.g. a NaN column): keep the diagonal
-        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
-        const int tp = p & 255, sp = p >> 8;
-        if (tid == tp) {
-#pragma unroll
-          for (int c = 0; c < W; ++c) {
-            T v = x[0][c];
-#pragma unroll
-            for (int s = 1; s < RP; ++s) v = bit_blend(v, x[s][c], s == sp);
-            prow[par][c] = v;
-          }
-        }
-        if (tid == j) {
-#pragma unroll
-          for (int c = 0; c < W; ++c) jrow[par][c] = x[0][c];
-        }
-        __syncthreads();
-        if (p != j) {
-          if (tid == tp) {
+          for (int s = 0; s < RP; ++s) own |= !dn[s] & (lp[s] == bi);
+          if (own & (best != 0ull)) {
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-              const T v = jrow[par][c];
+              T v = x[0][c];
 #pragma unroll
-              for (int s = 0; s < RP; ++s) x[s][c] = bit_blend(x[s][c], v, s == sp);
+              for (int s = 1; s < RP; ++s) v = bit_blend(v, x[s][c], !dn[s] & (lp[s] == bi));
+              cand[par][wave][c] = v;
             }
           }
-          if (tid == j) {
-#pragma unroll
-            for (int c = 0; c < W; ++c) x[0][c] = prow[par][c];
-          }
         }
+        __syncthreads();
+        // all wavefront keys first (one round of LDS reads), then a tree max
+        unsigned long long mk[NW];
+        int mw[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          mk[q] = red_k[par][q];
+          mw[q] = q;
+        }
+#pragma unroll
+        for (int st = 1; st < NW; st *= 2)
+#pragma unroll
+          for (int q = 0; q + st < NW; q += 2 * st) {
+            const bool take = mk[q + st] > mk[q];
+            mk[q] = take ? mk[q + st] : mk[q];
+            mw[q] = take ? mw[q + st] : mw[q];
+          }
+        const bool valid = mk[0] != 0ull;
+        // no candidate (a NaN column): keep the diagonal, results are NaN
+        const int p = valid ? pivot_key_index(mk[0], T(0)) : j;
+        const int wv = mw[0];
+        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
         T pr[W];
 #pragma unroll
-        for (int c = 0; c < W; ++c) pr[c] = prow[par][c];
+        for (int c = 0; c < W; ++c) pr[c] = cand[par][wv][c];
+#pragma unroll
+        for (int c = 0; c < W; ++c) pr[c] = bit_blend(T(NAN), pr[c], valid);
+#pragma unroll
+        for (int s = 0; s < RP; ++s) {
+          const int o = lp[s];
+          const bool piv = !dn[s] & (o == p);
+          lp[s] = dn[s] ? o : ((o == j) ? p : ((o == p) ? j : o));
+          dn[s] = dn[s] | piv;
+        }
         const T pv = pr[j];
         const bool nz = pv != T(0);
         if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
         const T dv = nz ? pv : T(1);
+        // FP64: one reciprocal per column (as LAPACK's getf2 scales by
+        // 1/pivot); FP32: true division, which refinement depends on
+        const T rv = sizeof(T) == 8 ? T(1) / dv : T(0);
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
-          const int rr = tid + 256 * s;
-          const bool m = (rr > j) & (rr < R);
-          const T l = x[s][j] / dv;
+          const bool m = !dn[s];
+          const T l = sizeof(T) == 8 ? x[s][j] * rv : x[s][j] / dv;
           const T lm = m ? l : T(0);
           x[s][j] = m ? l : x[s][j];
 #pragma unroll
@@ -586,103 +628,25 @@ static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
         }
       }
     });
-#pragma unroll
-    for (int s = 0; s < RP; ++s) {
-      const int rr = tid + 256 * s;
-      if (rr < R) {
-        T* rowp = A.at(b, k0 + rr, k0);
-#pragma unroll
-        for (int c = 0; c < W; ++c)
-          if (c < w) rowp[(int64_t)c * A.cs] = x[s][c];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// LU panel with one row per thread (R <= NT rows, NT = 256, 512 or 1024
-// threads, 4 to 16 wavefronts so that several share each SIMD and hide each
-// other's dependent-instruction latency).  Per column: wavefront argmax
-// (DPP), every wavefront merges the NT/64 partial results the same way, the
-// owners of the pivot row and of row j swap them through LDS, every thread
-// divides and updates its own row with the pivot row read from LDS.  Two
-// barriers per column; rows at or past npiv are never chosen as pivots.
-template <typename T, int W, int NT>
-__global__ __launch_bounds__(NT) void k_getf2_row(int batch, int nrows, int k0, int w, int npiv, View<T> A,
-                                                  int* ipiv, int64_t pbs, int* info) {
-  constexpr int NW = NT / 64;
-  __shared__ T prow[2][W];
-  __shared__ T jrow[2][W];
-  __shared__ T red_v[2][NW];
-  __shared__ int red_i[2][NW];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int R = nrows - k0, Rp = npiv - k0;
-  const int rm1 = R - 1, wm1 = w - 1;
-  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
-    T x[W];
-    {
-      const T* rowp = A.at(b, k0 + (tid < rm1 ? tid : rm1), k0);
-#pragma unroll
-      for (int c = 0; c < W; ++c) {
-        const T v = rowp[(int64_t)(c < wm1 ? c : wm1) * A.cs];
-        x[c] = (tid < R && c < w) ? v : T(0);
-      }
-    }
-    static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
-      constexpr int j = decltype(jc)::value;
-      if (j < w) {
-        const int par = j & 1;
-        const bool cand = (tid >= j) & (tid < Rp);
-        T best = cand ? tabs(x[j]) : T(-1);
-        int bi = cand ? tid : INT_MAX;
-        wave_argmax(best, bi);
-        if (lane == 0) {
-          red_v[par][wave] = best;
-          red_i[par][wave] = bi;
-        }
+    // coalesced store through the same tiles, each row to its final position
+    static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
+      constexpr int sl = decltype(sc)::value;
+      const int r0 = NT * sl + 64 * wave;
+      static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
+        constexpr int c0 = decltype(tc)::value * TW;
         __syncthreads();
-        best = red_v[par][lane % NW];
-        bi = red_i[par][lane % NW];
-        wave_argmax(best, bi);
-        int p = bi;
-        if (p < j || p >= Rp) p = j;  // no valid candidate (e.g. a NaN column): keep the diagonal
-        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
-        if (tid == p) {
 #pragma unroll
-          for (int c = 0; c < W; ++c) prow[par][c] = x[c];
-        }
-        if (tid == j) {
-#pragma unroll
-          for (int c = 0; c < W; ++c) jrow[par][c] = x[c];
-        }
+        for (int c = 0; c < TW; ++c) stage[wave][lane][c] = x[sl][c0 + c];
+        stage_pos[wave][lane] = lp[sl];
         __syncthreads();
-        if (p != j) {
-          if (tid == p) {
 #pragma unroll
-            for (int c = 0; c < W; ++c) x[c] = jrow[par][c];
-          }
-          if (tid == j) {
-#pragma unroll
-            for (int c = 0; c < W; ++c) x[c] = prow[par][c];
-          }
+        for (int q = 0; q < TW; ++q) {
+          const int e = 64 * q + lane;
+          const int rr = r0 + e / TW, cc = c0 + e % TW;
+          if (rr < R && cc < w) *A.at(b, k0 + stage_pos[wave][e / TW], k0 + cc) = stage[wave][e / TW][e % TW];
         }
-        const T pv = prow[par][j];
-        const bool nz = pv != T(0);
-        if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
-        const bool m = (tid > j) & (tid < R);
-        const T l = x[j] / (nz ? pv : T(1));
-        const T lm = m ? l : T(0);
-        x[j] = m ? l : x[j];
-#pragma unroll
-        for (int c = j + 1; c < W; ++c) x[c] -= lm * prow[par][c];
-      }
+      });
     });
-    if (tid < R) {
-      T* rowp = A.at(b, k0 + tid, k0);
-#pragma unroll
-      for (int c = 0; c < W; ++c)
-        if (c < w) rowp[(int64_t)c * A.cs] = x[c];
-    }
     __syncthreads();
   }
 }
@@ -782,20 +746,15 @@ __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, 
 
 static const size_t kPanelLdsBudget = 128 * 1024;
 
-// NT > 0: one row per thread in NT-thread workgroups; NT == 0: RP rows per
-// thread in 256-thread workgroups
+// RP rows per thread in NT-thread register panels
 template <typename T, int W, int RP, int NT>
 hipError_t getrf_reg(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs,
                      int* info, hipStream_t s) {
   hipError_t e;
   for (int k0 = 0; k0 < nelim; k0 += W) {
     const int w = (nelim - k0) < W ? (nelim - k0) : W;
-    if (NT > 0)
-      hipLaunchKernelGGL((k_getf2_row<T, W, (NT > 0 ? NT : 256)>), dim3(grid_batch(batch)), dim3(NT > 0 ? NT : 256),
-                         0, s, batch, nrows, k0, w, npiv, A, ipiv, pbs, info);
-    else
-      hipLaunchKernelGGL((k_getf2_reg<T, W, RP>), dim3(grid_batch(batch)), dim3(256), 0, s, batch, nrows, k0, w,
-                         npiv, A, ipiv, pbs, info);
+    hipLaunchKernelGGL((k_getf2_reg<T, W, RP, NT>), dim3(grid_batch(batch)), dim3(NT), 0, s, batch, nrows, k0, w,
+                       npiv, A, ipiv, pbs, info);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL((k_swap_trsm<T, W>), dim3(ceil_div(ncols, 64), grid_batch(batch)), dim3(64), 0, s, batch,
                        ncols, k0, w, A, ipiv, pbs);
@@ -814,15 +773,13 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
                  hipStream_t s) {
   if (batch <= 0 || nelim <= 0) return hipSuccess;
   if (nelim > npiv || npiv > nrows || nelim > ncols) return hipErrorInvalidValue;
-  // panels held in registers: up to 8 rows per thread (2048 rows)
-  // batched small matrices: 256-thread panels with up to 2 rows per thread;
-  // a single tall matrix: one row per thread in 1024-thread workgroups
-  if (nrows <= 256) return getrf_reg<T, 32, 1, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 512) return getrf_reg<T, 32, 2, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  // (8 FP64 or 16 FP32 columns per panel keep 1024 threads within 128 VGPRs)
-  if (nrows <= 1024)
-    return getrf_reg<T, sizeof(T) == 8 ? 8 : 16, 1, 1024>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 2048) return getrf_reg<T, 8, 8, 0>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  // panels held in registers, RP rows per thread: batched small matrices in
+  // 256-thread workgroups (several per CU), up to 1024 rows in 512-thread
+  // workgroups (two wavefronts per SIMD), four rows per thread up to 2048
+  if (nrows <= 256) return getrf_reg<T, 32, 1, 256>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 512) return getrf_reg<T, 32, 2, 256>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 1024) return getrf_reg<T, 16, 2, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 2048) return getrf_reg<T, 8, 4, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   // taller panels: LDS-resident panel, generic interchanges
   int nb = 32;
   while (nb > 4 && (size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) nb >>= 1;

@@ -66,6 +66,51 @@ __device__ inline void wave_argmax(T& v, int& i) {
   i = i0;
 }
 
+// Pivot search keys: one unsigned 64-bit number per candidate whose
+// unsigned order is (|a| ascending, then row index descending), so a plain
+// maximum picks the largest magnitude and, among equal magnitudes, the
+// smallest index (LAPACK i?amax order).  FP32: |a| bits above the inverted
+// 32-bit index (exact).  FP64: |a| bits with the low 12 mantissa bits
+// replaced by the inverted index (indices below 2048; magnitudes that agree
+// to 2^-40 relative count as equal).  0 means no candidate; the sign of a
+// is ignored, a must not be NaN.
+__device__ inline unsigned long long pivot_key(float a, int idx) {
+  return ((unsigned long long)(__float_as_uint(a) & 0x7FFFFFFFu) << 32) |
+         (unsigned long long)(0xFFFFFFFFu - (unsigned)idx);
+}
+__device__ inline unsigned long long pivot_key(double a, int idx) {
+  return ((unsigned long long)__double_as_longlong(a) & 0x7FFFFFFFFFFFF000ull) | (unsigned long long)(0xFFF - idx);
+}
+__device__ inline int pivot_key_index(unsigned long long k, float) { return (int)(0xFFFFFFFFu - (unsigned)k); }
+__device__ inline int pivot_key_index(unsigned long long k, double) { return 0xFFF - (int)(k & 0xFFFull); }
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ inline unsigned long long dpp_u64(unsigned long long v) {
+  // lanes outside ROWMASK keep their own value
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const unsigned rlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+  const unsigned rhi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+  return ((unsigned long long)rhi << 32) | rlo;
+}
+
+__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+
+// All-reduce maximum of a 64-bit key over the wavefront with VALU-only
+// steps: quad permutations and row mirrors reduce each 16-lane row, then
+// row_bcast:15 / row_bcast:31 fold the rows into lane 63, which is read
+// once.
+__device__ inline unsigned long long wave_max_u64(unsigned long long k) {
+  k = umax64(k, dpp_u64<0xB1>(k));        // quad_perm [1,0,3,2]
+  k = umax64(k, dpp_u64<0x4E>(k));        // quad_perm [2,3,0,1]
+  k = umax64(k, dpp_u64<0x141>(k));       // row_half_mirror
+  k = umax64(k, dpp_u64<0x140>(k));       // row_mirror
+  k = umax64(k, dpp_u64<0x142, 0xA>(k));  // row_bcast:15 into rows 1 and 3
+  k = umax64(k, dpp_u64<0x143, 0xC>(k));  // row_bcast:31 into rows 2 and 3
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 // Bitwise blend: (a & ~m) | (b & m) on the bit patterns, m all ones or all
 // zeros.  Picking one of several register-resident values this way keeps
 // every index compile-time (a select chain would be folded into a load

@@ -98,7 +98,7 @@ def _lu_check(Af, LUf, piv, n):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("n", [1, 5, 20, 33, 100, 257])
+@pytest.mark.parametrize("n", [1, 5, 20, 33, 100, 257, 700, 1000])
 def test_getrf_square_matches_lapack(dtype, n):
     hip = _hip()
     g = torch.Generator().manual_seed(n)
