@@ -80,8 +80,24 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
       if (k == 3) return a.sup + b * a.ubs + (int64_t)i * a.uis;
       return a.diag + b * a.dbs + (int64_t)i * a.dis;
     };
+    // forward sweeps start at the first block whose right-hand side tile
+    // has a nonzero: the blocks above it solve to zero and already hold
+    // zeros (a block-sparse right-hand side, as in the slab Schur
+    // complements, skips most of its forward sweep)
+    int i0 = 0;
+    if (mode != 2) {
+      for (; i0 < m; ++i0) {
+        bool nz = false;
+        for (int e = tid; e < nf; e += 256) {
+          const int r = e >> lgnr, c = e & (nr - 1);
+          if (c < ncols) nz |= *(frow(i0, r) + c) != T(0);
+        }
+        if (__syncthreads_or(nz)) break;
+      }
+      if (i0 == m) continue;  // zero right-hand side: zero solution, already in place
+    }
     auto f_row = [&](int k, int i) -> int {  // block row of f loaded into Xc
-      if (k == 0) return i == 0 ? 0 : -1;
+      if (k == 0) return i == i0 ? i : -1;
       if (k == 1) return i + 1;
       if (k == 3) return i;
       return -1;
@@ -157,10 +173,11 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
         Y[r * ldx + c] = (c < ncols) ? *(frow(m, r) + c) : T(0);
       }
     }
-    prefetch(0);
-    commit(0);
+    const int s0 = mode == 2 ? 0 : 2 * i0;
+    prefetch(s0);
+    commit(s0);
     __syncthreads();
-    for (int s = 0; s < nst; ++s) {
+    for (int s = s0; s < nst; ++s) {
       if (s + 1 < nst) prefetch(s + 1);
       int i;
       const int k = decode(s, i);

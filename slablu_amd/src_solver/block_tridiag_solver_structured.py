@@ -36,13 +36,18 @@ def _dense(x, device):
 
 
 def _inverse(S):
-    """S (b, n, n) -> S^{-1} (b, n, n) through getrf + getrs on the identity."""
+    """S (b, n, n) -> S^{-1} (b, n, n) = U^{-1} L^{-1} P from getrf; P (the
+    row interchanges applied to the identity) is written directly from the
+    composed permutation, then the two triangular solves run on it."""
     b, n, _ = S.shape
     ipiv = torch.empty(b, n, dtype=torch.int32, device=S.device)
     info = torch.zeros(b, dtype=torch.int32, device=S.device)
     hip.getrf(S, ipiv, info=info)
-    X = torch.eye(n, dtype=S.dtype, device=S.device).expand(b, n, n).contiguous()
-    hip.getrs(S, ipiv, X)
+    perm = hip.piv_to_perm(ipiv)
+    X = torch.zeros(b, n, n, dtype=S.dtype, device=S.device)
+    X.scatter_(2, perm.long().unsqueeze(-1), 1.0)            # row r of P is e_{perm[r]}
+    hip.trsm(S, X, lower=True, unit=True)
+    hip.trsm(S, X, lower=False, unit=False)
     return X, info
 
 

@@ -201,6 +201,40 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
     assert _rel(f, ref) < 200 * TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("twisted", [False, True])
+@pytest.mark.parametrize("nrhs", [1, 70])
+def test_btri_solve_block_sparse_rhs(dtype, twisted, nrhs):
+    """Right-hand sides whose leading blocks are zero (a different first
+    nonzero block per column, one all-zero 32-column tile): the forward
+    sweeps start at the first nonzero block and must give the dense
+    solution."""
+    hip = _hip()
+    b, m, bs = 2, 11, 24
+    g = torch.Generator().manual_seed(7 + nrhs)
+    diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
+    sub0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+    sup0 = torch.randn(b, m - 1, bs, bs, generator=g, dtype=torch.float64) * 0.3
+    f0 = torch.zeros(b, m, bs, nrhs, dtype=torch.float64)
+    for c in range(nrhs):
+        if 32 <= c < 64:
+            continue                                   # an all-zero column tile
+        first = (c * 5 + 3) % m
+        f0[:, first:min(m, first + 2), :, c] = torch.randn(b, min(m, first + 2) - first, bs, generator=g,
+                                                          dtype=torch.float64)
+    f0 = f0.reshape(b, m * bs, nrhs)
+    diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
+    ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
+    hip.btri_factor(diag, sub, sup, ipiv, twisted=twisted)
+    f = f0.to("cuda", dtype).contiguous()
+    hip.btri_solve(diag, sub, sup, ipiv, f, twisted=twisted)
+    ref = torch.linalg.solve(_btri_dense(diag0, sub0, sup0), f0)
+    torch.cuda.synchronize()
+    assert _rel(f, ref) < 200 * TOL[dtype]
+    if nrhs > 64:
+        assert torch.count_nonzero(f[:, :, 32:64]) == 0
+
+
 @pytest.mark.parametrize("p", [10, 22])
 @pytest.mark.parametrize("general", [False, True])
 @pytest.mark.parametrize("nrhs", [1, 3])
