@@ -478,10 +478,11 @@ class Domain_Solver:
         rel = (torch.linalg.norm(res) / torch.linalg.norm(ff.double())).item()
         return x_face, rel, toc
 
-    def _body_reduce(self, fb):
+    def _body_reduce(self, fb, return_solve=False):
         """Face load -sum_b P_b N_c A_cc^{-1} f_b of interior loads fb
         (nboxes, ni, nrhs) (the reduction reduce_body intends,
-        hps_parallel_leaf_ops.py:94-100)."""
+        hps_parallel_leaf_ops.py:94-100); with return_solve, also the leaf
+        solves A_cc^{-1} f_b."""
         ops, lf = self.hps.ops, self.hps.leaf
         ni = ops.ni
         w = fb.to(lf["dtype"]).contiguous().clone()
@@ -490,7 +491,7 @@ class Domain_Solver:
         hip.gemm(ops.typed(w.dtype)["Nx_c"], w, z, alpha=-1.0)
         y = torch.zeros(self.hps.nface, fb.shape[-1], dtype=w.dtype, device=w.device)
         y.index_add_(0, self.hps.Iext_d.reshape(-1), z.reshape(-1, fb.shape[-1]))
-        return y
+        return (y, w) if return_solve else y
 
     def solve(self, uu_dir_func, ff_body_func=None, known_sol=False):
         """Returns (sol_tot at all leaf nodes (cpu), rel_err, true_err,
@@ -533,13 +534,15 @@ class Domain_Solver:
         Returns (dI (nboxes, ni, nc), dC (|I_Ctot|, nc)) in FP64."""
         hps, ops, lf = self.hps, self.hps.ops, self.hps.leaf
         Ic = self._Ic()
-        ff = rF + self._body_reduce(rI)[Ic].double()
+        yb, w = self._body_reduce(rI, return_solve=True)
+        ff = rF + yb[Ic].double()
         dC = self.solve_reduced(ff.to(self.wdt)).double()
         dF = torch.zeros(hps.nface, dC.shape[-1], dtype=torch.float64, device=self.device)
         dF[Ic] = dC
         db = hps.faces_to_boxes(dF).to(self.wdt).contiguous()
-        dI = ops.interior(lf["M"], lf["ipiv"], db, rI.to(self.wdt).contiguous(), perm=lf.get("perm"))
-        return dI.double(), dC
+        # A_cc^{-1}(r_I - A_cx d_x) = w - S' d_x with the stored S' = A_cc^{-1} A_cx
+        hip.gemm(lf["M"][:, :, ops.ni:], db, w, alpha=-1.0, beta=1.0)
+        return w.double(), dC
 
     def _apply_full(self, dI, dC):
         """A d of the un-condensed system (FP64) for a vector with zero
