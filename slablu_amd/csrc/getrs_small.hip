@@ -206,21 +206,58 @@ __global__ __launch_bounds__(256) void k_getrs_wave(int batch, int n, int nrhs, 
       for (int c = 0; c < NC; ++c) X[(d0 + lane) * NC + c] = x[c];
     }
     __builtin_amdgcn_wave_barrier();
+    // rows outside the diagonal block: X[r] -= LU[r, d0:d0+tb] . X[d0:d0+tb].
+    // Sixteen lanes share a row (four consecutive columns each), so a load
+    // instruction reads four row segments contiguously; the lane's four
+    // solution entries stay in registers and each row's sum is a DPP
+    // reduction within its 16-lane row.  Sixteen (FP32) or eight (FP64) row
+    // groups per iteration keep as many loads per lane in flight.
     const int r0 = lower ? d0 + tb : 0;
     const int r1 = lower ? n : d0;
-    const T* Xb = X + d0 * NC;
-    for (int r = r0 + lane; r < r1; r += 64) {
-      T l[64];
-      load_seg<T>(LU.at(b, r, d0), vec, tb, l);
-      T acc[NC];
+    const int ro = lane >> 4, cc = (lane & 15) * 4;
+    T xs[4][NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc[c] = T(0);
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int j = 0; j < 64; ++j)
+      for (int c = 0; c < NC; ++c) {
+        const T v = X[(d0 + (cc + u < tb ? cc + u : tb - 1)) * NC + c];
+        xs[u][c] = cc + u < tb ? v : T(0);
+      }
+    const bool vseg = vec && (tb % 4) == 0;
+    const int ccl = cc < tb ? cc : 0;  // column chunk inside the block (masked by xs)
+    constexpr int U = sizeof(T) == 4 ? 16 : 8;
+    const int rm1 = r1 - 1;
+    for (int rb = r0; rb < r1; rb += 4 * U) {
+      T l[U][4];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] += l[j] * Xb[(j < tb ? j : tb - 1) * NC + c];
+      for (int g = 0; g < U; ++g) {
+        const int r = rb + 4 * g + ro;
+        const T* rowp = LU.at(b, r < rm1 ? r : rm1, d0);
+        if (vseg) {
+          typedef T v4_t __attribute__((ext_vector_type(4)));
+          const v4_t v = *reinterpret_cast<const v4_t*>(rowp + ccl);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) X[r * NC + c] -= acc[c];
+          for (int u = 0; u < 4; ++u) l[g][u] = v[u];
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) l[g][u] = rowp[cc + u < tb ? cc + u : tb - 1];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < U; ++g) {
+        const int r = rb + 4 * g + ro;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          T acc = l[g][0] * xs[0][c];
+#pragma unroll
+          for (int u = 1; u < 4; ++u) acc += l[g][u] * xs[u][c];
+          acc += dpp<0xB1>(acc);   // quad_perm [1,0,3,2]
+          acc += dpp<0x4E>(acc);   // quad_perm [2,3,0,1]
+          acc += dpp<0x141>(acc);  // row_half_mirror
+          acc += dpp<0x140>(acc);  // row_mirror
+          if ((lane & 15) == 0 && r < r1) X[r * NC + c] -= acc;
+        }
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
