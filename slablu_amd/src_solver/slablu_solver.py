@@ -619,10 +619,11 @@ class Domain_Solver:
                 pack(wI, wF, w)
                 h = torch.zeros(nc, j + 2, dtype=torch.float64, device=dev)
                 for _ in range(2):                                  # classical Gram-Schmidt, twice
-                    c = (V[:, :j + 1] * w.unsqueeze(1)).sum(-1)     # (nc, j+1)
-                    for i in range(j + 1):
-                        w.addcmul_(V[:, i], c[:, i:i + 1], value=-1.0)
-                    h[:, :j + 1] += c
+                    for col in range(nc):                           # matrix-vector products on the basis
+                        Vb = V[col, :j + 1]                         # (j+1, ntot), rows contiguous
+                        c = torch.mv(Vb, w[col])
+                        w[col].addmv_(Vb.t(), c, alpha=-1.0)
+                        h[col, :j + 1] += c
                 h[:, j + 1] = torch.linalg.norm(w, dim=1)
                 V[:, j + 1] = w / h[:, j + 1].clamp_min(1e-300)[:, None]
                 H[:, :j + 2, j] = h.cpu().numpy()
