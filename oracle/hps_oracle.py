@@ -209,8 +209,9 @@ def helmholtz_pdo(kh, bfield=bfield_constant):
 class HPSOracle:
     """HPS_Multidomain restated (src_disc/hps_multidomain_disc.py:17-250)."""
 
-    def __init__(self, pdo, domain, a, p):
+    def __init__(self, pdo, domain, a, p, periodic=False):
         self.pdo, self.p, self.a = pdo, p, a
+        self.periodic = periodic
         self.domain = np.asarray(domain, dtype=np.float64)
         n = (self.domain[:, 1] - self.domain[:, 0]) / (2 * a)
         self.n = np.round(n).astype(int)
@@ -337,7 +338,10 @@ class HPSOracle:
             for j in range(n1):
                 b = i * n1 + j
                 I[b, :f] += (off + j) * f
-                I[b, f:2 * f] += (off + 2 * n1 + 1 + j) * f
+                # periodic in x: the last column's right faces are the
+                # domain's left boundary faces (slablu_solver.py:91-104)
+                rface = j if (self.periodic and i == n0 - 1) else off + 2 * n1 + 1 + j
+                I[b, f:2 * f] += rface * f
                 I[b, 2 * f:3 * f] += (off + n1 + j) * f
                 I[b, 3 * f:] += (off + n1 + j + 1) * f
         return I
@@ -350,7 +354,9 @@ class HPSOracle:
         nb, ne = I.shape
         rows = np.repeat(I, ne, axis=1).reshape(nb, ne, ne)
         cols = np.transpose(rows, (0, 2, 1))
-        A = scipy.sparse.coo_matrix((DtN.flatten(), (rows.flatten(), cols.flatten()))).tocsr()
+        f = self.p - 2
+        nface = f * (self.n0 * (2 * self.n1 + 1) + self.n1)
+        A = scipy.sparse.coo_matrix((DtN.flatten(), (rows.flatten(), cols.flatten())), shape=(nface, nface)).tocsr()
         return A
 
     def leaf_solve(self, uu_sol, DtN_unused=None, ff_body_func=None, chunk=256):
@@ -421,8 +427,9 @@ class SlabPartition:
     I_Ctot, I_Xtot for the non-periodic HPS case
     (slablu_solver.py:61-124, 127-202)."""
 
-    def __init__(self, n0, n1, p, buf_constant):
+    def __init__(self, n0, n1, p, buf_constant, periodic=False):
         f = p - 2
+        self.periodic = periodic
         self.f, self.n0, self.n1 = f, n0, n1
         buf = slab_buf(n0, n1, p, buf_constant)
         Npan = n0 // buf
@@ -452,14 +459,25 @@ class SlabPartition:
         self.I_D = I_D.flatten()
         self.I_U = I_U.flatten()
         self.I_C = np.hstack((I_elim.flatten(), I_rem.flatten()))
-        self.I_slabC = self.inds_pans[:, self.I_C].flatten()
-        self.I_slabX = self.inds_pans[1:, self.I_L].flatten()
-        self.I_Ctot = np.hstack((self.I_slabC, self.I_slabX))
         I_Ldir = self.inds_pans[0, self.I_L]
-        I_Rdir = self.inds_pans[Npan - 1, self.I_R]
+        I_Rdir = self.inds_pans[Npan - 1, self.I_R].copy()
         I_Ddir = self.inds_pans[:, self.I_D].flatten()
         I_Udir = self.inds_pans[:, self.I_U].flatten()
-        self.I_Xtot = np.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+        self.I_seam_copy = I_Rdir       # face ids of the domain's right boundary
+        if periodic:
+            # the last slab's right faces are slab 0's left faces; every slab's
+            # left faces are an interface (slablu_solver.py:91-123)
+            if Npan < 2:
+                raise ValueError("periodic slabs need at least 2 slabs")
+            self.inds_pans[Npan - 1, self.I_R] = self.inds_pans[0, self.I_L]
+            self.I_slabC = self.inds_pans[:, self.I_C].flatten()
+            self.I_slabX = self.inds_pans[:, self.I_L].flatten()
+            self.I_Xtot = np.hstack((I_Ddir, I_Udir))
+        else:
+            self.I_slabC = self.inds_pans[:, self.I_C].flatten()
+            self.I_slabX = self.inds_pans[1:, self.I_L].flatten()
+            self.I_Xtot = np.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+        self.I_Ctot = np.hstack((self.I_slabC, self.I_slabX))
 
 
 def dof_count(p, n0, n1):
@@ -503,6 +521,9 @@ def full_solve(hps, part, uu_dir_func, known_sol=True, A=None, DtN=None, x_C=Non
     sol_face = np.zeros((A.shape[0], ff.shape[1]))
     sol_face[part.I_Ctot] = x_C
     sol_face[part.I_Xtot] = uu_dir
+    if getattr(part, "periodic", False):
+        # the right-boundary face slots carry the seam (left-boundary) values
+        sol_face[part.I_seam_copy] = sol_face[part.inds_pans[0, part.I_L]]
     sol_tot, resloc = hps.leaf_solve(sol_face)
     true_err = float("nan")
     if known_sol:
@@ -743,12 +764,22 @@ class SlabLUOracle:
 
 
 def setup_problem(n, p, ppw=None, kh=None, domain=((0.0, 1.0), (0.0, 1.0)), bfield=bfield_constant,
-                  buf_constant=1.0):
-    """Driver parameter derivation (argparse_driver.py:86-163)."""
+                  buf_constant=1.0, periodic=False):
+    """Driver parameter derivation (argparse_driver.py:86-163); periodic:
+    x-periodic, Dirichlet data on the bottom and top boundaries only."""
     if kh is None:
         kh = kh_from_ppw(n, ppw)
     npan = n / (p - 2)
     a = 1 / (2 * npan)
-    hps = HPSOracle(helmholtz_pdo(kh, bfield), np.asarray(domain), a, p)
-    part = SlabPartition(hps.n0, hps.n1, p, buf_constant)
+    hps = HPSOracle(helmholtz_pdo(kh, bfield), np.asarray(domain), a, p, periodic=periodic)
+    part = SlabPartition(hps.n0, hps.n1, p, buf_constant, periodic=periodic)
     return hps, part, kh
+
+
+def uu_periodic_mode(xx, kh, m=2):
+    """cos(2 pi m x) cos(k_y y), k_y^2 = kh^2 - (2 pi m)^2: an exact solution
+    of -Lap u - kh^2 u = 0 that is 1-periodic in x (a closed form for the
+    periodic known-solution check)."""
+    xx = np.asarray(xx)
+    ky = np.sqrt(kh ** 2 - (2 * np.pi * m) ** 2)
+    return (np.cos(2 * np.pi * m * xx[:, 0]) * np.cos(ky * xx[:, 1]))[:, None]

@@ -27,8 +27,9 @@ def _default_device():
 
 
 class HPS_Multidomain:
-    def __init__(self, pdo, domain, a, p, device=None):
+    def __init__(self, pdo, domain, a, p, device=None, periodic=False):
         self.pdo = pdo
+        self.periodic = periodic
         self.domain = torch.as_tensor(domain, dtype=torch.float64)
         self.p, self.a = p, a
         n = (self.domain[:, 1] - self.domain[:, 0]) / (2 * a)
@@ -91,7 +92,13 @@ class HPS_Multidomain:
         i = torch.arange(n0).repeat_interleave(n1)
         j = torch.arange(n1).repeat(n0)
         off = (2 * n1 + 1) * i
-        faces = torch.stack((off + j, off + 2 * n1 + 1 + j, off + n1 + j, off + n1 + j + 1), dim=1)
+        right = off + 2 * n1 + 1 + j
+        if self.periodic:
+            # periodic in x: the last column's right faces are the domain's
+            # left boundary faces (slablu_solver.py:91-104), so the face
+            # operator couples across the seam
+            right = torch.where(i == n0 - 1, j, right)
+        faces = torch.stack((off + j, right, off + n1 + j, off + n1 + j + 1), dim=1)
         return (faces[:, :, None] * f + torch.arange(f)[None, None, :]).reshape(self.nboxes, 4 * f)
 
     # -- device state -----------------------------------------------------------

@@ -54,13 +54,17 @@ def _inverse(S):
 class HBS_BlockTridiagonal:
     def __init__(self, b, n, cyclic=False):
         assert b < n
-        if cyclic:
-            raise NotImplementedError("periodic (cyclic) interface systems are not provided")
+        if cyclic and b < 2:
+            raise ValueError("a cyclic block-tridiagonal system needs at least 2 blocks")
         self.b, self.n, self.cyclic = b, n, cyclic
 
     def build_lowmem(self, hbs_diag0, hbs_diag1, hbs_sub, hbs_sup, device=None, dtype=torch.float64):
-        """diag = diag0 + diag1 (b, n, n); sub, sup (b-1, n, n); dense
-        tensors or objects with todense()."""
+        """diag = diag0 + diag1 (b, n, n); sub, sup (b-1, n, n), with sub[j]
+        the block (j+1, j) and sup[j] the block (j, j+1); dense tensors or
+        objects with todense().  Cyclic: sub, sup (b, n, n), sub[b-1] the
+        corner block (0, b-1) and sup[b-1] the corner block (b-1, 0)."""
+        if self.cyclic:
+            return self._build_cyclic(hbs_diag0, hbs_diag1, hbs_sub, hbs_sup, device, dtype)
         device = torch.device("cuda") if device is None else torch.device(device)
         m, n = self.b, self.n
         diag = _dense(hbs_diag0, device).to(dtype)
@@ -122,8 +126,65 @@ class HBS_BlockTridiagonal:
 
     build = build_lowmem
 
+    def _build_cyclic(self, hbs_diag0, hbs_diag1, hbs_sub, hbs_sup, device, dtype):
+        """Cyclic system (the periodic interface ring, block_tridiag_solver_
+        structured.py:118-155) as an acyclic chain over blocks 0..b-2 with a
+        border at block b-1:
+            [T  B] [x ]   [f ]      B: blocks (0, b-1) and (b-2, b-1)
+            [C  D] [xb] = [fb],     C: blocks (b-1, 0) and (b-1, b-2)
+        W = T^{-1} B by the chain solver, then the Schur complement
+        S = D - C W (FP64, explicit inverse)."""
+        device = torch.device("cuda") if device is None else torch.device(device)
+        m, n = self.b, self.n
+        diag = _dense(hbs_diag0, device).to(dtype)
+        if hbs_diag1 is not None:
+            diag = diag + _dense(hbs_diag1, device).to(dtype)
+        diag = diag.reshape(m, n, n)
+        sub = _dense(hbs_sub, device).to(dtype).reshape(m, n, n)
+        sup = _dense(hbs_sup, device).to(dtype).reshape(m, n, n)
+        self.inner = HBS_BlockTridiagonal(m - 1, n)
+        self.inner.build_lowmem(diag[:m - 1], None, sub[:m - 2] if m > 2 else None,
+                                sup[:m - 2] if m > 2 else None, device=device, dtype=dtype)
+        # border column B and row C (blocks may coincide when b = 2)
+        Bcol = torch.zeros(m - 1, n, n, dtype=dtype, device=device)
+        Bcol[0] += sub[m - 1]
+        Bcol[m - 2] += sup[m - 2]
+        self.C0 = sup[m - 1].contiguous()        # block (b-1, 0)
+        self.Cl = sub[m - 2].contiguous()        # block (b-1, b-2)
+        W = self.inner.solve(Bcol.reshape((m - 1) * n, n)).reshape(m - 1, n, n)
+        S = diag[m - 1].clone().unsqueeze(0)
+        hip.gemm(self.C0, W[0].unsqueeze(0), S, alpha=-1.0, beta=1.0)
+        hip.gemm(self.Cl, W[m - 2].unsqueeze(0), S, alpha=-1.0, beta=1.0)
+        self.Sinv, info = _inverse(S)
+        self.W = W.reshape((m - 1) * n, n).contiguous()
+        self.info = torch.maximum(self.inner.info, torch.zeros_like(self.inner.info))
+        self.info[0] = torch.maximum(self.info[0], info[0])
+        self.nbytes = self.inner.nbytes + sum(t.numel() * t.element_size()
+                                              for t in (self.Sinv, self.W, self.C0, self.Cl))
+        self.dtype, self.device = dtype, device
+        self.build_mode = "lowmem"
+
+    def _solve_cyclic(self, f):
+        m, n = self.b, self.n
+        nrhs = f.shape[-1]
+        fb = f.to(self.device, self.dtype).reshape(m * n, nrhs)
+        y = self.inner.solve(fb[:(m - 1) * n]).reshape(m - 1, n, nrhs)
+        r = fb[(m - 1) * n:].clone().unsqueeze(0)
+        hip.gemm(self.C0, y[0].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+        hip.gemm(self.Cl, y[m - 2].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+        xb = torch.empty(1, n, nrhs, dtype=self.dtype, device=self.device)
+        hip.gemm(self.Sinv, r, xb)
+        x = torch.empty(m * n, nrhs, dtype=self.dtype, device=self.device)
+        xs = y.reshape((m - 1) * n, nrhs).clone().unsqueeze(0)
+        hip.gemm(self.W.unsqueeze(0), xb, xs, alpha=-1.0, beta=1.0)
+        x[:(m - 1) * n] = xs[0]
+        x[(m - 1) * n:] = xb[0]
+        return x
+
     def solve(self, f):
         """f (b*n, nrhs) -> solution (b*n, nrhs); f is not modified."""
+        if self.cyclic:
+            return self._solve_cyclic(f)
         m, n, k, L = self.b, self.n, self.k, self.L
         nrhs = f.shape[-1]
         fb = f.to(self.device, self.dtype).reshape(m, n, nrhs)

@@ -72,8 +72,6 @@ class Domain_Solver:
                  device=None, refine_tol=1e-9, refine_maxit=12, refine_method="fgmres", refine_restart=10):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
-        if periodic_bc:
-            raise NotImplementedError("periodic boundary conditions are not provided")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.kh = kh
@@ -92,14 +90,25 @@ class Domain_Solver:
         self.disc = "hps"
         self.ntot = self.hps.xx_active.shape[0]
         I_L, I_C, Npan = self.I_L, self.I_C, self.Npan
-        self.I_slabC = self.inds_pans[:, I_C].flatten()
-        self.I_slabX = self.inds_pans[1:, I_L].flatten()
-        self.I_Ctot = torch.hstack((self.I_slabC, self.I_slabX))
         I_Ldir = self.inds_pans[0, I_L]
         I_Rdir = self.inds_pans[Npan - 1, self.I_R]
         I_Ddir = self.inds_pans[:, self.I_D].flatten()
         I_Udir = self.inds_pans[:, self.I_U].flatten()
-        self.I_Xtot = torch.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+        if periodic_bc:
+            # x-periodic (slablu_solver.py:91-123): the last slab's right
+            # faces are slab 0's left faces, every slab's left faces form an
+            # interface, Dirichlet data only on the bottom and top
+            if Npan < 2:
+                raise ValueError("periodic boundary conditions need at least 2 slabs")
+            self.inds_pans[Npan - 1, self.I_R] = self.inds_pans[0, I_L]
+            self.I_slabC = self.inds_pans[:, I_C].flatten()
+            self.I_slabX = self.inds_pans[:, I_L].flatten()
+            self.I_Xtot = torch.hstack((I_Ddir, I_Udir))
+        else:
+            self.I_slabC = self.inds_pans[:, I_C].flatten()
+            self.I_slabX = self.inds_pans[1:, I_L].flatten()
+            self.I_Xtot = torch.hstack((I_Ldir, I_Rdir, I_Ddir, I_Udir))
+        self.I_Ctot = torch.hstack((self.I_slabC, self.I_slabX))
 
     def _Ic(self):
         """I_Ctot on the device (cached: no host-to-device copy per call)."""
@@ -110,7 +119,8 @@ class Domain_Solver:
     # ------------------------------------------------------------------ setup
     def hps_disc(self, box_geom, a, p, pdo_op):
         """Leaf grid and slab width (slablu_solver.py:127-163)."""
-        H = hps_multidomain_disc.HPS_Multidomain(pdo_op, box_geom, a, p, device=self.device)
+        H = hps_multidomain_disc.HPS_Multidomain(pdo_op, box_geom, a, p, device=self.device,
+                                                 periodic=self.periodic_bc)
         f = p - 2
         n0, n1 = H.n0, H.n1
         n_tmp = max(n0, n1) * f - 1
@@ -219,8 +229,12 @@ class Domain_Solver:
 
     def interface_blocks(self):
         """A restricted to each slab interface (block diagonal over the leaf
-        rows): (Npan-1, n1*f, n1*f)."""
+        rows): (Npan-1, n1*f, n1*f); periodic: (Npan, ...) with the seam
+        (interface 0, the left faces of slab 0) first."""
         Np, buf = self.Npan, self.buf_pans
+        if self.periodic_bc:
+            left = torch.roll(self.Dt7[:, buf - 1, :, R_, :, R_, :], 1, dims=0)
+            return _blockdiag(left + self.Dt7[:, 0, :, L_, :, L_, :])
         return _blockdiag(self.Dt7[:Np - 1, buf - 1, :, R_, :, R_, :] + self.Dt7[1:, 0, :, L_, :, L_, :])
 
     def build_Aloc_solver(self):
@@ -339,14 +353,23 @@ class Domain_Solver:
         if self.stage_sync:
             torch.cuda.synchronize()
             self.t_Tform = time.time() - self._t_tred0
-        # interface s between slab s and s+1
         Aii = self.interface_blocks()
         KR = Kf[:, L:, L:].to(torch.float64)
         KL = Kf[:, :L, :L].to(torch.float64)
-        diag = Aii.to(torch.float64) - KR[:Np - 1] - KL[1:]
-        sup = -Kf[1:Np - 1, :L, L:].to(torch.float64)
-        sub = -Kf[1:Np - 1, L:, :L].to(torch.float64)
-        self.Tred_solver = HBS_BlockTridiagonal(Np - 1, L)
+        if self.periodic_bc:
+            # interface j = left faces of slab j (j = 0 is the seam); slab j
+            # couples interface j to interface j+1 (mod Npan)
+            # (slablu_solver.py:1267-1305)
+            diag = Aii.to(torch.float64) - torch.roll(KR, 1, dims=0) - KL
+            sup = -Kf[:, :L, L:].to(torch.float64)
+            sub = -Kf[:, L:, :L].to(torch.float64)
+            self.Tred_solver = HBS_BlockTridiagonal(Np, L, cyclic=True)
+        else:
+            # interface s between slab s and s+1
+            diag = Aii.to(torch.float64) - KR[:Np - 1] - KL[1:]
+            sup = -Kf[1:Np - 1, :L, L:].to(torch.float64)
+            sub = -Kf[1:Np - 1, L:, :L].to(torch.float64)
+            self.Tred_solver = HBS_BlockTridiagonal(Np - 1, L)
         self.Tred_solver.build_lowmem(diag, None, sub, sup, device=dev, dtype=torch.float64)
 
     # ------------------------------------------------------------------ solve
@@ -430,13 +453,23 @@ class Domain_Solver:
         if Np == 1:
             return yC.reshape(Np * nC, nc)
         Z = self._AXC(ue, upad)
-        ffX = ff[Np * nC:].view(Np - 1, n1, f, nc).to(torch.float64) - Z[:Np - 1, :, 1].double() \
-            - Z[1:, :, 0].double()
-        solX = self.Tred_solver.solve(ffX.reshape((Np - 1) * L, nc))
-        sx = solX.view(Np - 1, n1, f, nc)
-        x = torch.zeros(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
-        x[1:, :, 0] = sx.to(ff.dtype)
-        x[:Np - 1, :, 1] = sx.to(ff.dtype)
+        if self.periodic_bc:
+            # interface j sees slab j-1 from the left and slab j from the right
+            ffX = ff[Np * nC:].view(Np, n1, f, nc).to(torch.float64) \
+                - torch.roll(Z[:, :, 1], 1, dims=0).double() - Z[:, :, 0].double()
+            solX = self.Tred_solver.solve(ffX.reshape(Np * L, nc))
+            sx = solX.view(Np, n1, f, nc).to(ff.dtype)
+            x = torch.empty(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
+            x[:, :, 0] = sx
+            x[:, :, 1] = torch.roll(sx, -1, dims=0)
+        else:
+            ffX = ff[Np * nC:].view(Np - 1, n1, f, nc).to(torch.float64) - Z[:Np - 1, :, 1].double() \
+                - Z[1:, :, 0].double()
+            solX = self.Tred_solver.solve(ffX.reshape((Np - 1) * L, nc))
+            sx = solX.view(Np - 1, n1, f, nc)
+            x = torch.zeros(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
+            x[1:, :, 0] = sx.to(ff.dtype)
+            x[:Np - 1, :, 1] = sx.to(ff.dtype)
         solC, _, _ = self.solve_slab(ffC - self._ACX(x.view(Np, n1, 2 * f, nc)))
         return torch.cat((solC.reshape(Np * nC, nc), solX.to(ff.dtype)), dim=0)
 

@@ -130,3 +130,45 @@ def test_mixed_refined_vs_fp64_oracle():
     print("mixed: rel err vs FP64 oracle", err, "refinement", dom.refine_history, "true_err", true_err, flush=True)
     assert err < 1e-8
     assert abs(true_err - r["true_err"]) < 1e-9
+
+
+def _periodic(n, p, ppw, precision, bufc=0.6, m=2):
+    """x-periodic Helmholtz on the unit square (n/(p-2) leaves per side, so
+    the leaves tile the period exactly), Dirichlet data on the bottom and top
+    from the exact mode cos(2 pi m x) cos(k_y y)."""
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    kh = O.kh_from_ppw(n, ppw)
+    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_constant(xx, kh))
+    a = 1 / (2 * (n / (p - 2)))
+    dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, a, p=p, buf_constant=bufc,
+                        periodic_bc=True, precision=precision)
+    uu = lambda xx: torch.as_tensor(O.uu_periodic_mode(np.asarray(xx), kh, m))
+    hps, part, _ = O.setup_problem(n, p, kh=kh, periodic=True, buf_constant=bufc)
+    r = O.full_solve(hps, part, lambda xx: O.uu_periodic_mode(xx, kh, m))
+    return dom, uu, r
+
+
+@pytest.mark.parametrize("n,p", [(64, 10), (128, 10), (112, 16)])
+def test_periodic_fp64_vs_oracle(n, p):
+    """Periodic slabs (cyclic interface ring) against the oracle's SuperLU
+    solve of the same x-periodic discrete system, and its known-solution
+    error."""
+    dom, uu, r = _periodic(n, p, 10, "fp64")
+    print("periodic case", n, p, "Npan", dom.Npan, flush=True)
+    dom.build(verbose=False)
+    sol, res, true_err, resloc, toc = dom.solve(uu, known_sol=True)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("periodic fp64 rel err vs oracle", err, "true_err", true_err, r["true_err"], flush=True)
+    assert err < 1e-10
+    assert abs(true_err - r["true_err"]) < 1e-10
+    assert res < 1e-10
+
+
+def test_periodic_mixed_vs_oracle():
+    dom, uu, r = _periodic(128, 10, 10, "mixed")
+    dom.build(verbose=False)
+    sol, res, true_err, resloc, toc = dom.solve(uu, known_sol=True)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("periodic mixed rel err vs oracle", err, "refinement", dom.refine_history, flush=True)
+    assert err < 1e-8
