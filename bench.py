@@ -77,7 +77,7 @@ def measured_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def make_problem(n, p, ppw, precision):
+def make_problem(n, p, ppw, precision, periodic=False):
     import torch
     from slablu_amd.src_disc import pdo, built_in_funcs as bif
     from slablu_amd.src_solver.slablu_solver import Domain_Solver
@@ -86,7 +86,7 @@ def make_problem(n, p, ppw, precision):
     op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_lshaped_waveguide(xx, kh))
     a = 1 / (2 * (n / (p - 2)))
     dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
-                        precision=precision)
+                        precision=precision, periodic_bc=periodic)
     uu_dir = lambda xx: bif.uu_dir_func_greens(xx, kh)
     return dom, uu_dir, kh
 
@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--ppw", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--periodic", action="store_true",
+                    help="x-periodic slabs (cyclic interface ring), Dirichlet data on bottom and top only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SlabLU path on a single MI355X "
@@ -186,7 +188,7 @@ def main():
     torch.set_num_threads(len(os.sched_getaffinity(0)))
     n, p, ppw = args.n, args.p, args.ppw
     print("bench: mixed path n=%d p=%d" % (n, p), file=sys.stderr, flush=True)
-    dom, uu_dir, kh = make_problem(n, p, ppw, "mixed")
+    dom, uu_dir, kh = make_problem(n, p, ppw, "mixed", args.periodic)
     N = O.dof_count(p, dom.hps.n0, dom.hps.n1)
     ts, tf, tsol, leaf_ms, sol_mixed, stages = run_path(dom, uu_dir, args.steps, args.warmup, leaf_events=True)
     refine_hist = list(dom.refine_history)
@@ -197,7 +199,7 @@ def main():
     fp64 = None
     if not args.no_fp64:
         print("bench: fp64 path", file=sys.stderr, flush=True)
-        dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64")
+        dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64", args.periodic)
         ts64, tf64, tsol64, leaf64, sol64, stages64 = run_path(dom64, uu_dir, max(1, args.steps), args.warmup,
                                                                 leaf_events=True)
         err = (torch.linalg.norm(sol_mixed - sol64) / torch.linalg.norm(sol64)).item()
@@ -225,9 +227,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "Helmholtz, L-shaped waveguide bumps, unit square, n=%d p=%d ppw=%d kh=%.2f, "
-                               "%dx%d leaves, N=%d, mixed precision + FP64 refinement"
-                               % (n, p, ppw, kh, int(math.sqrt(nboxes)), int(math.sqrt(nboxes)), N),
-                   "N": N, "p": p, "leaves": nboxes, "precision": "mixed"},
+                               "%dx%d leaves, N=%d, mixed precision + FP64 refinement%s"
+                               % (n, p, ppw, kh, int(math.sqrt(nboxes)), int(math.sqrt(nboxes)), N,
+                                  ", x-periodic" if args.periodic else ""),
+                   "N": N, "p": p, "leaves": nboxes, "precision": "mixed", "periodic": bool(args.periodic)},
         "factor_ms": 1e3 * sum(tf) / len(tf), "solve_ms": 1e3 * sum(tsol) / len(tsol),
         "refine_corrections": refine_hist, "stages_ms": stages,
         "fp64": fp64,
