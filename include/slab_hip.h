@@ -173,6 +173,18 @@ int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D
                        const double* c1, const double* c2, const double* c,
                        const double* Nxtot, void* M, int64_t ldm, int64_t m_bs, void* stream);
 
+/* Interior rows [A_cc | A_c,X] of every leaf where the columns past the
+ * (p-2)^2 interior ones are the nxc leaf nodes xnodes (node id i*p + j),
+ * e.g. the 4(p-1) corner-inclusive exterior nodes in Jxreorder order, which
+ * the condensation needs when the cross term c12 couples interior and
+ * corner nodes.  Coefficients and accumulation as slab_leaf_assemble.
+ * Replaces the c12 branch of form_DtNs, Aloc[Jc][:, Jxreorder]
+ * (src_disc/hps_parallel_leaf_ops.py:59-65). */
+int slab_leaf_assemble_nodes(int dtype, int nboxes, int p, int nxc, const int* xnodes, const double* D,
+                             const double* Dsq, const double* c11, const double* c22, const double* c12,
+                             const double* c1, const double* c2, const double* c, void* M, int64_t ldm, int64_t m_bs,
+                             void* stream);
+
 /* FP64 residual of the un-condensed leaf collocation system for every leaf
  * in one launch (matrix-free): node values from interior values uc
  * (nboxes, (p-2)^2, nrhs) at the Jc nodes and per-leaf face values ub

@@ -292,6 +292,22 @@ int slab_leaf_assemble(int dtype, int nboxes, int p, int border, const double* D
   return -1;
 }
 
+int slab_leaf_assemble_nodes(int dtype, int nboxes, int p, int nxc, const int* xnodes, const double* D,
+                             const double* Dsq, const double* c11, const double* c22, const double* c12,
+                             const double* c1, const double* c2, const double* c, void* M, int64_t ldm, int64_t m_bs,
+                             void* stream) {
+  if (nboxes < 0 || p < 3 || nxc < 1 || nxc > p * p || !xnodes || !D || !Dsq || !c11 || !c22) return -1;
+  const int f = p - 2;
+  if (ldm < (int64_t)f * f + nxc || m_bs < (int64_t)f * f * ldm) return -1;
+  if (dtype == SLAB_F64)
+    return status(leaf_assemble<double>(nboxes, p, 0, D, Dsq, c11, c22, c12, c1, c2, c, nullptr,
+                                        reinterpret_cast<double*>(M), ldm, m_bs, S(stream), xnodes, nxc));
+  if (dtype == SLAB_F32)
+    return status(leaf_assemble<float>(nboxes, p, 0, D, Dsq, c11, c22, c12, c1, c2, c, nullptr,
+                                       reinterpret_cast<float*>(M), ldm, m_bs, S(stream), xnodes, nxc));
+  return -1;
+}
+
 int slab_leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
                        const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
                        const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,

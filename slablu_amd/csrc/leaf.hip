@@ -36,17 +36,20 @@ __device__ inline void leaf_col_node(int col, int ni, int f, int p, int& j1, int
   }
 }
 
+// xnodes (optional): node ids (i*p + j) of the columns past the interior
+// ones, nxc of them, e.g. the 4(p-1) corner-inclusive exterior nodes in
+// Jxreorder order; NULL: the 4(p-2) face nodes in [l, r, d, u] order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nrows, int rows_per_block,
                                                        const double* __restrict__ D, const double* __restrict__ Dsq,
                                                        const double* c11, const double* c22, const double* c12,
                                                        const double* c1, const double* c2, const double* c,
                                                        const double* __restrict__ Nxtot, T* M, int64_t ldm,
-                                                       int64_t mbs) {
+                                                       int64_t mbs, const int* __restrict__ xnodes, int nxc) {
   // separate multiply and add roundings, as the reference's
   // `Aloc += f_vals * D` accumulation does (no fused multiply-add)
 #pragma clang fp contract(off)
-  const int f = p - 2, ni = f * f, nx = 4 * f, ncols = ni + nx;
+  const int f = p - 2, ni = f * f, nx = xnodes ? nxc : 4 * f, ncols = ni + nx;
   const int b = blockIdx.x;
   if (b >= nboxes) return;
   const int r0 = blockIdx.y * rows_per_block;
@@ -67,7 +70,13 @@ __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nr
       const double v0 = c ? c[ci] : 0.0;
       for (int col = threadIdx.x; col < ncols; col += blockDim.x) {
         int j1, j2;
-        leaf_col_node(col, ni, f, p, j1, j2);
+        if (xnodes && col >= ni) {
+          const int node = xnodes[col - ni];
+          j1 = node / p;
+          j2 = node % p;
+        } else {
+          leaf_col_node(col, ni, f, p, j1, j2);
+        }
         double val = 0.0;
         val += v11 * (i2 == j2 ? Dsq[i1 * p + j1] : 0.0);
         val += v22 * (i1 == j1 ? Dsq[i2 * p + j2] : 0.0);
@@ -87,22 +96,23 @@ __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nr
 template <typename T>
 hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const double* Dsq, const double* c11,
                          const double* c22, const double* c12, const double* c1, const double* c2, const double* c,
-                         const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s) {
+                         const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s, const int* xnodes,
+                         int nxc) {
   const int f = p - 2, ni = f * f, nx = 4 * f;
   const int nrows = ni + (border ? nx : 0);
   const int rpb = 8;
   dim3 grid(nboxes, (nrows + rpb - 1) / rpb);
   hipLaunchKernelGGL(k_leaf_assemble<T>, grid, dim3(256), 0, s, nboxes, p, nrows, rpb, D, Dsq, c11, c22, c12, c1,
-                     c2, c, Nxtot, M, ldm, mbs);
+                     c2, c, Nxtot, M, ldm, mbs, xnodes, nxc);
   return hipGetLastError();
 }
 
 template hipError_t leaf_assemble<double>(int, int, int, const double*, const double*, const double*, const double*,
                                           const double*, const double*, const double*, const double*, const double*,
-                                          double*, int64_t, int64_t, hipStream_t);
+                                          double*, int64_t, int64_t, hipStream_t, const int*, int);
 template hipError_t leaf_assemble<float>(int, int, int, const double*, const double*, const double*, const double*,
                                          const double*, const double*, const double*, const double*, const double*,
-                                         float*, int64_t, int64_t, hipStream_t);
+                                         float*, int64_t, int64_t, hipStream_t, const int*, int);
 
 // FP64 residual of the un-condensed collocation system, matrix-free, one
 // workgroup per leaf.  The leaf's p^2 node values are assembled in LDS from

@@ -64,7 +64,8 @@ hipError_t leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t 
 template <typename T>
 hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const double* Dsq, const double* c11,
                          const double* c22, const double* c12, const double* c1, const double* c2, const double* c,
-                         const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s);
+                         const double* Nxtot, T* M, int64_t ldm, int64_t mbs, hipStream_t s,
+                         const int* xnodes = nullptr, int nxc = 0);
 }  // namespace slab
 
 namespace slab {

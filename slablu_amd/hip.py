@@ -53,6 +53,8 @@ def _load():
                                  _i64, _i64, _int, _vp, _vp], _int),
         "slab_leaf_assemble": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                 _vp], _int),
+        "slab_leaf_assemble_nodes": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _i64, _i64, _vp], _int),
         "slab_leaf_residual": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp],
                                _int),
@@ -289,6 +291,24 @@ def leaf_assemble(M, p, border, D, Dsq, coefs, Nxtot):
                  "coefficient arrays must be contiguous FP64 of nboxes*p*p")
     _check(LIB.slab_leaf_assemble(_dt(M), M.shape[0], p, int(border), _p(D), _p(Dsq), *[_p(a) for a in args],
                                   _p(Nxtot), _p(M), M.stride(1), M.stride(0), _stream()), "slab_leaf_assemble")
+    return M
+
+
+def leaf_assemble_nodes(M, p, xnodes, D, Dsq, coefs):
+    """Fill M (nboxes, (p-2)^2, >= (p-2)^2 + len(xnodes)) with [A_cc | A_c,X]
+    of every leaf, the X columns at the leaf nodes xnodes (int32 device
+    tensor of node ids i*p + j)."""
+    _require(M.dim() == 3 and M.stride(2) == 1, "leaf M layout")
+    _require(xnodes.dtype == torch.int32 and xnodes.is_contiguous(), "xnodes must be contiguous int32")
+    for k in ("c11", "c22"):
+        _require(coefs.get(k) is not None, "coefficient %s required" % k)
+    args = [coefs.get(k) for k in ("c11", "c22", "c12", "c1", "c2", "c")]
+    for a in args:
+        _require(a is None or (a.dtype == torch.float64 and a.is_contiguous() and a.numel() == M.shape[0] * p * p),
+                 "coefficient arrays must be contiguous FP64 of nboxes*p*p")
+    _check(LIB.slab_leaf_assemble_nodes(_dt(M), M.shape[0], p, xnodes.numel(), _p(xnodes), _p(D), _p(Dsq),
+                                        *[_p(a) for a in args], _p(M), M.stride(1), M.stride(0), _stream()),
+           "slab_leaf_assemble_nodes")
     return M
 
 

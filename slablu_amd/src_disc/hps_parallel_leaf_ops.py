@@ -47,6 +47,7 @@ class LeafOps:
         # N_x in compressed rows of p entries (each row is a derivative along
         # one grid line), for the fused residual kernel
         Nx = np.asarray(H.Nx, dtype=np.float64)
+        self.Nx_nat = t(Nx)
         nx_idx = np.zeros((Nx.shape[0], p), dtype=np.int32)
         nx_val = np.zeros((Nx.shape[0], p), dtype=np.float64)
         for r in range(Nx.shape[0]):
@@ -68,6 +69,7 @@ class LeafOps:
                 Nx_c=self.Nxtot[:, :ni].to(dtype).contiguous(),
                 Nx_x=self.Nxtot[:, ni:].to(dtype).contiguous(),
                 Intmap=self.Intmap.to(dtype).contiguous(),
+                NxI=(self.Nx_nat[:, self.Jxreo] @ self.Intmap).to(dtype).contiguous(),
                 D=self.D.to(dtype).contiguous(),
                 Dsq=self.Dsq.to(dtype).contiguous(),
             )
@@ -79,14 +81,31 @@ class LeafOps:
         tensor.  Returns (DtN (nboxes, nx, nx), M (nboxes, ni, ni+nx) factored,
         ipiv (nboxes, ni) int32, info (nboxes,) int32)."""
         ni, nx = self.ni, self.nx
-        M = torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
-        hip.leaf_assemble(M, self.p, 0, self.D, self.Dsq, coefs, self.Nxtot)
         ipiv = torch.empty(nboxes, ni, dtype=torch.int32, device=self.device)
         info = torch.zeros(nboxes, dtype=torch.int32, device=self.device)
-        hip.getrf(M, ipiv, nelim=ni, npiv=ni, info=info)
-        hip.trsm(M[:, :, :ni], M[:, :, ni:], lower=False, unit=False)
         ty = self.typed(dtype)
-        DtN = ty["Nx_x"].expand(nboxes, nx, nx).contiguous()
+        if coefs.get("c12") is None:
+            M = torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
+            hip.leaf_assemble(M, self.p, 0, self.D, self.Dsq, coefs, self.Nxtot)
+            hip.getrf(M, ipiv, nelim=ni, npiv=ni, info=info)
+            hip.trsm(M[:, :, :ni], M[:, :, ni:], lower=False, unit=False)
+            DtN = ty["Nx_x"].expand(nboxes, nx, nx).contiguous()
+        else:
+            # the cross term couples interior and corner nodes: condense
+            # against the corner-inclusive exterior nodes (Jxreorder) and
+            # map their values from the face values through Intmap
+            # (src_disc/hps_parallel_leaf_ops.py:59-65):
+            #   S' = A_cc^{-1} A[Jc, Jxr] Intmap,  DtN = Nx[:, Jxr] Intmap - Nx_c S'
+            nxr = self.Jxreo.numel()
+            Mr = torch.empty(nboxes, ni, ni + nxr, dtype=dtype, device=self.device)
+            hip.leaf_assemble_nodes(Mr, self.p, self.res_tables["Jxreo"], self.D, self.Dsq, coefs)
+            hip.getrf(Mr, ipiv, nelim=ni, npiv=ni, info=info)
+            hip.trsm(Mr[:, :, :ni], Mr[:, :, ni:], lower=False, unit=False)
+            M = torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
+            M[:, :, :ni] = Mr[:, :, :ni]
+            hip.gemm(Mr[:, :, ni:], ty["Intmap"], M[:, :, ni:])
+            del Mr
+            DtN = ty["NxI"].expand(nboxes, nx, nx).contiguous()
         hip.gemm(ty["Nx_c"], M[:, :, ni:], DtN, alpha=-1.0, beta=1.0)
         return DtN, M, ipiv, info
 

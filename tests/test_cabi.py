@@ -15,7 +15,7 @@ def _declared():
 def test_header_declares_entry_points():
     names = _declared()
     for must in ("slab_gemm", "slab_getrf", "slab_btri_factor", "slab_btri_solve", "slab_btri_inv_factor",
-                 "slab_btri_inv_solve", "slab_leaf_assemble", "slab_leaf_residual", "slab_btri_factor_twisted", "slab_btri_solve_twisted",
+                 "slab_btri_inv_solve", "slab_leaf_assemble", "slab_leaf_assemble_nodes", "slab_leaf_residual", "slab_btri_factor_twisted", "slab_btri_solve_twisted",
                  "slab_btri_fused"):
         assert must in names
 

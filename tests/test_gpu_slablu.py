@@ -172,3 +172,54 @@ def test_periodic_mixed_vs_oracle():
     err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
     print("periodic mixed rel err vs oracle", err, "refinement", dom.refine_history, flush=True)
     assert err < 1e-8
+
+
+def _curved(geom, n, p, ppw, box, periodic, precision="fp64"):
+    """Helmholtz on a mapped domain (annulus sector, or the full curvy
+    annulus with x-periodic slabs), Dirichlet data = the free-space Green's
+    function at the mapped nodes, as the reference's test_2d.py drives it."""
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    kh = O.kh_from_ppw(n, ppw)
+    op, zmap, _ = pdo.get_param_map_and_pdo(geom, bif.bfield_constant, kh)
+    a = 1 / (2 * (n / (p - 2)))
+    dom = Domain_Solver(torch.tensor([[0, box[0]], [0, box[1]]]), op, kh, a, p=p, buf_constant=0.6,
+                        periodic_bc=periodic, precision=precision)
+    uu = lambda xx: bif.uu_dir_func_greens(zmap(torch.as_tensor(xx)), kh)
+    npf = lambda f: None if f is None else (lambda xx: f(torch.as_tensor(xx)).numpy().reshape(-1))
+    opo = O.PDO(npf(op.c11), npf(op.c22), c12=npf(op.c12), c1=npf(op.c1), c2=npf(op.c2), c=npf(op.c))
+    hps = O.HPSOracle(opo, np.array([[0, box[0]], [0, box[1]]]), a, p, periodic=periodic)
+    part = O.SlabPartition(hps.n0, hps.n1, p, 0.6, periodic=periodic)
+    r = O.full_solve(hps, part, lambda xx: np.asarray(uu(xx)).reshape(-1, 1))
+    return dom, uu, r, hps
+
+
+@pytest.mark.parametrize("geom,box,periodic", [("annulus", (1.0, 1.0), False), ("sinusoidal", (1.0, 1.0), False),
+                                               ("curvy_annulus", (6.0, 1.0), True)])
+def test_curved_domain_fp64_vs_oracle(geom, box, periodic):
+    """Mapped domains with the c12 cross term (corner-inclusive leaf
+    condensation through Interp_mat) against the oracle's SuperLU solve of
+    the same discrete system; the curvy annulus is the reference's
+    test_helm_poisson_curvyannulus configuration (box 6 x 1, periodic) at a
+    smaller n."""
+    dom, uu, r, hps = _curved(geom, 64, 10, 40, box, periodic)
+    print("curved case", geom, "Npan", dom.Npan, flush=True)
+    dom.build(verbose=False)
+    DtN = dom.hps.leaf["DtN"].double().cpu().numpy()
+    DtN_o = hps.dtns()
+    assert np.abs(DtN - DtN_o).max() < 1e-9 * np.abs(DtN_o).max()
+    sol, res, true_err, resloc, toc = dom.solve(uu, known_sol=True)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print(geom, "fp64 rel err vs oracle", err, "true_err", true_err, r["true_err"], flush=True)
+    assert err < 1e-10
+    assert abs(true_err - r["true_err"]) < 1e-10
+    assert true_err < 1e-3
+
+
+def test_curved_domain_mixed_vs_oracle():
+    dom, uu, r, _ = _curved("annulus", 64, 10, 40, (1.0, 1.0), False, precision="mixed")
+    dom.build(verbose=False)
+    sol, res, true_err, resloc, toc = dom.solve(uu, known_sol=True)
+    err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
+    print("annulus mixed rel err vs oracle", err, "refinement", dom.refine_history, flush=True)
+    assert err < 1e-8
