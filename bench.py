@@ -77,17 +77,18 @@ def measured_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def make_problem(n, p, ppw, precision, periodic=False):
+def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.0, pde="waveguide"):
     import torch
     from slablu_amd.src_disc import pdo, built_in_funcs as bif
     from slablu_amd.src_solver.slablu_solver import Domain_Solver
     nwaves = int(n / ppw)
     kh = (nwaves + 0.03) * 2 * math.pi + 1.8
-    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_lshaped_waveguide(xx, kh))
+    bfield = bif.bfield_lshaped_waveguide if pde == "waveguide" else bif.bfield_constant
+    op, zmap, _ = pdo.get_param_map_and_pdo(domain, bfield, kh)
     a = 1 / (2 * (n / (p - 2)))
-    dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
+    dom = Domain_Solver(torch.tensor([[0, box_x], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
                         precision=precision, periodic_bc=periodic)
-    uu_dir = lambda xx: bif.uu_dir_func_greens(xx, kh)
+    uu_dir = lambda xx: bif.uu_dir_func_greens(zmap(torch.as_tensor(xx)), kh)
     return dom, uu_dir, kh
 
 
@@ -176,6 +177,11 @@ def main():
     ap.add_argument("--ppw", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--domain", default="square", choices=["square", "sinusoidal", "annulus", "curvy_annulus"],
+                    help="reference rectangle mapped to this domain (the c12 cross term for curved ones)")
+    ap.add_argument("--box-x", type=float, default=1.0, help="width of the reference rectangle")
+    ap.add_argument("--pde", default="waveguide", choices=["waveguide", "constant"],
+                    help="coefficient b(x): L-shaped waveguide bumps, or constant (kh^2)")
     ap.add_argument("--periodic", action="store_true",
                     help="x-periodic slabs (cyclic interface ring), Dirichlet data on bottom and top only")
     args = ap.parse_args()
@@ -188,18 +194,20 @@ def main():
     torch.set_num_threads(len(os.sched_getaffinity(0)))
     n, p, ppw = args.n, args.p, args.ppw
     print("bench: mixed path n=%d p=%d" % (n, p), file=sys.stderr, flush=True)
-    dom, uu_dir, kh = make_problem(n, p, ppw, "mixed", args.periodic)
+    extra = dict(periodic=args.periodic, domain=args.domain, box_x=args.box_x, pde=args.pde)
+    dom, uu_dir, kh = make_problem(n, p, ppw, "mixed", **extra)
     N = O.dof_count(p, dom.hps.n0, dom.hps.n1)
     ts, tf, tsol, leaf_ms, sol_mixed, stages = run_path(dom, uu_dir, args.steps, args.warmup, leaf_events=True)
     refine_hist = list(dom.refine_history)
     nboxes = dom.hps.nboxes
+    n0, n1 = dom.hps.n0, dom.hps.n1
     roof_dom = time_leaf_solve(dom)
     del dom
     torch.cuda.empty_cache()
     fp64 = None
     if not args.no_fp64:
         print("bench: fp64 path", file=sys.stderr, flush=True)
-        dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64", args.periodic)
+        dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64", **extra)
         ts64, tf64, tsol64, leaf64, sol64, stages64 = run_path(dom64, uu_dir, max(1, args.steps), args.warmup,
                                                                 leaf_events=True)
         err = (torch.linalg.norm(sol_mixed - sol64) / torch.linalg.norm(sol64)).item()
@@ -226,9 +234,12 @@ def main():
         "metric": METRIC, "value": N / t_step, "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": "Helmholtz, L-shaped waveguide bumps, unit square, n=%d p=%d ppw=%d kh=%.2f, "
+        "config": {"workload": "Helmholtz, %s, %s, n=%d p=%d ppw=%d kh=%.2f, "
                                "%dx%d leaves, N=%d, mixed precision + FP64 refinement%s"
-                               % (n, p, ppw, kh, int(math.sqrt(nboxes)), int(math.sqrt(nboxes)), N,
+                               % ("L-shaped waveguide bumps" if args.pde == "waveguide" else "constant coefficient",
+                                  "unit square" if args.domain == "square" and args.box_x == 1.0
+                                  else "%s domain (reference box %g x 1)" % (args.domain, args.box_x),
+                                  n, p, ppw, kh, n0, n1, N,
                                   ", x-periodic" if args.periodic else ""),
                    "N": N, "p": p, "leaves": nboxes, "precision": "mixed", "periodic": bool(args.periodic)},
         "factor_ms": 1e3 * sum(tf) / len(tf), "solve_ms": 1e3 * sum(tsol) / len(tsol),
