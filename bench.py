@@ -9,7 +9,7 @@ on a 0.04 lattice with one row and one column removed (L-shaped waveguide),
 Dirichlet data from the free-space Green's function.  A step is one full
 factor + solve of the mixed-precision path (FP32 leaf/slab stages, FP64
 interface LU, FP64-residual refinement of the full discrete system until the face
-solution changes by less than 1e-9, relative).  The FP64 path is timed on the
+solution changes by less than 1e-8, relative).  The FP64 path is timed on the
 same problem and its solution is the reference for the error of the mixed
 solution.
 
