@@ -549,15 +549,18 @@ class Domain_Solver:
         return uu.double().cpu(), rel, true_err, resloc, toc
 
     # ------------------------------------------------------- mixed precision
-    def full_residual(self, x_face, uc, uu=None):
+    def full_residual(self, x_face, uc, uu=None, fb=None):
         """FP64 residual of the un-condensed system at the iterate given by
         the face vector x_face (Dirichlet values included) and the leaf
         interior values uc (nboxes, ni, nc): r_I = f - A_loc u at the leaf
-        interiors, r_F = -(sum of outward fluxes) at the face DOFs I_Ctot.
-        One fused leaf launch; uu, if given, receives all node values."""
+        interiors (f = fb, the body load at the interior nodes, or zero),
+        r_F = -(sum of outward fluxes) at the face DOFs I_Ctot.  One fused
+        leaf launch; uu, if given, receives all node values."""
         hps = self.hps
         nc = x_face.shape[-1]
         rI, fl = hps.ops.residual(uc, hps.faces_to_boxes(x_face), hps.coefs, uu=uu)
+        if fb is not None:
+            rI += fb
         y = torch.zeros(hps.nface, nc, dtype=torch.float64, device=self.device)
         y.index_add_(0, hps.Iext_d.reshape(-1), fl.reshape(-1, nc))
         return rI, -y[self._Ic()]
@@ -587,11 +590,11 @@ class Domain_Solver:
         rI, rF = self.full_residual(x, dI)
         return -rI, -rF
 
-    def _refine_richardson(self, x_face, uc):
+    def _refine_richardson(self, x_face, uc, fb=None):
         """x <- x + M^{-1}(b - A x) until the face correction is small."""
         Ic = self._Ic()
         for it in range(self.refine_maxit):
-            rI, rF = self.full_residual(x_face, uc)
+            rI, rF = self.full_residual(x_face, uc, fb=fb)
             dI, dC = self._correction(rI, rF)
             x_face[Ic] += dC
             uc += dI
@@ -600,7 +603,7 @@ class Domain_Solver:
             if rel_corr <= self.refine_tol:
                 break
 
-    def _refine_fgmres(self, x_face, uc):
+    def _refine_fgmres(self, x_face, uc, fb=None):
         """Flexible GMRES on the full system, right-preconditioned by the FP32
         two-level solve M: the Arnoldi relation A Z_j = V_{j+1} H_j holds in
         FP64 for the stored preconditioned directions Z_j = M^{-1} V_j, so the
@@ -635,7 +638,7 @@ class Domain_Solver:
         steps = 0
         done = False
         while steps < self.refine_maxit and not done:
-            rI, rF = self.full_residual(x_face, uc)
+            rI, rF = self.full_residual(x_face, uc, fb=fb)
             V = torch.empty(nc, kmax + 1, ntot, dtype=torch.float64, device=dev)
             Z = torch.empty(nc, kmax, ntot, dtype=torch.float64, device=dev)
             pack(rI, rF, V[:, 0])
@@ -691,8 +694,6 @@ class Domain_Solver:
         """Mixed precision: FP32 factors, FP64 residuals of the full system,
         flexible GMRES (or plain refinement) until the face solution changes
         by at most refine_tol (relative)."""
-        if ff_body_func is not None:
-            raise NotImplementedError("body loads with precision='mixed' are not provided")
         hps, ops = self.hps, self.hps.ops
         dev = self.device
         lf = hps.leaf
@@ -702,17 +703,22 @@ class Domain_Solver:
         x_lift = x_face.clone()
         nc = uu_dir.shape[1]
         uc = torch.zeros(hps.nboxes, ops.ni, nc, dtype=torch.float64, device=dev)
+        # body load f at the leaf interior nodes (FP64), part of the
+        # un-condensed residual r_I = f - A_loc u
+        fb = None if ff_body_func is None else hps.body_load(ff_body_func, nc, torch.float64)
         self.refine_history = []
         if self.refine_method == "fgmres":
-            self._refine_fgmres(x_face, uc)
+            self._refine_fgmres(x_face, uc, fb=fb)
         else:
-            self._refine_richardson(x_face, uc)
+            self._refine_richardson(x_face, uc, fb=fb)
         uu = torch.empty(hps.nboxes, ops.p * ops.p, nc, dtype=torch.float64, device=dev)
-        rI, rF = self.full_residual(x_face, uc, uu=uu)
+        rI, rF = self.full_residual(x_face, uc, uu=uu, fb=fb)
         torch.cuda.synchronize()
         toc = time.time() - tic
         resloc = torch.linalg.norm(rI).item()
         ff0 = -self._face_residual(x_lift, lf["DtN"]).double()
+        if fb is not None:
+            ff0 = ff0 + self._body_reduce(fb)[self._Ic()].double()
         rel = (torch.linalg.norm(rF) / torch.linalg.norm(ff0)).item()
         self.last_face = x_face
         return uu.reshape(-1, nc), rel, toc, resloc

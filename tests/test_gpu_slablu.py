@@ -223,3 +223,30 @@ def test_curved_domain_mixed_vs_oracle():
     err = np.linalg.norm(sol.numpy() - r["sol_tot"]) / np.linalg.norm(r["sol_tot"])
     print("annulus mixed rel err vs oracle", err, "refinement", dom.refine_history, flush=True)
     assert err < 1e-8
+
+
+def test_body_load_manufactured_solution():
+    """Body loads, FP64 and mixed: u = sin(a x) sin(b y) solves
+    -Lap u - kh^2 u = (a^2 + b^2 - kh^2) u; both paths recover it to
+    discretization error and agree with each other to the mixed tolerance."""
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    n, p, ppw = 128, 10, 10
+    kh = O.kh_from_ppw(n, ppw)
+    ca, cb = 9.0, 13.0
+    exact = lambda xx: (torch.sin(ca * torch.as_tensor(xx)[:, 0]) * torch.sin(cb * torch.as_tensor(xx)[:, 1]))[:, None]
+    body = lambda xx: (ca ** 2 + cb ** 2 - kh ** 2) * exact(xx)
+    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_constant(xx, kh))
+    a = 1 / (2 * (n / (p - 2)))
+    sols = {}
+    for precision in ("fp64", "mixed"):
+        dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, a, p=p, buf_constant=0.6,
+                            precision=precision)
+        dom.build(verbose=False)
+        sol, res, true_err, resloc, toc = dom.solve(exact, ff_body_func=body, known_sol=True)
+        print("body load", precision, "true_err", true_err, flush=True)
+        assert true_err < 1e-5
+        sols[precision] = sol
+    err = torch.linalg.norm(sols["mixed"] - sols["fp64"]) / torch.linalg.norm(sols["fp64"])
+    print("body load mixed vs fp64", err.item(), flush=True)
+    assert err < 1e-8
