@@ -65,6 +65,16 @@ int slab_getrs(int dtype, int batch, int n, int nrhs, int trans,
                const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv, int64_t p_bs,
                void* B, int64_t ldb, int64_t b_bs, void* stream);
 
+/* Batched inverse X = A^{-1} (n x n, out of place) from the factors of
+ * slab_getrf: both triangles inverted by recursive doubling of their
+ * diagonal blocks, then X = U^{-1} L^{-1} P.  Replaces torch.linalg.lu_factor
+ * + lu_solve of each interface Schur complement against its coupling block
+ * (block_tridiag_solver_structured.py:46-65): with the inverse kept, every
+ * later solve is a product.  ws: device workspace of slab_getri_ws bytes. */
+size_t slab_getri_ws(int dtype, int batch, int n);
+int slab_getri(int dtype, int batch, int n, const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv,
+               int64_t p_bs, void* X, int64_t ldx, int64_t x_bs, void* ws, void* stream);
+
 /* perm[r] (0-based) = the row of B that the interchanges ipiv (1-based, as
  * slab_getrf writes them) bring to row r; batched, perm row stride q_bs.
  * Lets repeated solves with one factorization skip replaying the swaps. */

@@ -148,6 +148,23 @@ int slab_piv_to_perm(int batch, int n, const int* ipiv, int64_t p_bs, int* perm,
   return status(piv_to_perm_batched(batch, n, ipiv, p_bs, perm, q_bs, S(stream)));
 }
 
+size_t slab_getri_ws(int dtype, int batch, int n) {
+  if (batch < 0 || n < 0 || (dtype != SLAB_F64 && dtype != SLAB_F32)) return 0;
+  return getri_ws_bytes(batch, n, dtype == SLAB_F64 ? sizeof(double) : sizeof(float));
+}
+
+int slab_getri(int dtype, int batch, int n, const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv,
+               int64_t p_bs, void* X, int64_t ldx, int64_t x_bs, void* ws, void* stream) {
+  if (batch < 0 || n < 0 || (batch > 0 && n > 0 && ws == nullptr)) return -1;
+  if (dtype == SLAB_F64)
+    return status(getri<double>(batch, n, view<double>(const_cast<void*>(LU), lda, 1, lu_bs), ipiv, p_bs,
+                                view<double>(X, ldx, 1, x_bs), ws, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(getri<float>(batch, n, view<float>(const_cast<void*>(LU), lda, 1, lu_bs), ipiv, p_bs,
+                               view<float>(X, ldx, 1, x_bs), ws, S(stream)));
+  return -1;
+}
+
 int slab_getrs_perm(int dtype, int batch, int n, int nrhs, const void* LU, int64_t lda, int64_t lu_bs,
                     const int* perm, int64_t q_bs, void* B, int64_t ldb, int64_t b_bs, void* stream) {
   if (batch < 0 || n < 1 || nrhs < 1 || nrhs > 8) return -1;

@@ -48,8 +48,12 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
-                                              T beta, View<T> C, int tiles_n) {
+                                              T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co) {
   typedef typename Mfma<T>::acc_t acc_t;
+  // second batch level (grid z): whole-operand offsets
+  A.p += blockIdx.z * ao;
+  B.p += blockIdx.z * bo;
+  C.p += blockIdx.z * co;
   __shared__ T As[GBK][GBM + 1];
   __shared__ T Bs[GBK][GBN + 1];
   const int tile = blockIdx.x;
@@ -180,6 +184,20 @@ __global__ __launch_bounds__(256) void k_gemv(int batch, int m, int n, int k, T 
   }
 }
 
+// Two batch levels: `outer` groups of `batch` products, group g offset by
+// g * (ao, bo, co) elements from the views.
+template <typename T>
+hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, int m, int n, int k, T alpha,
+                      View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
+  if (outer <= 0 || batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  if (outer > 65535) return hipErrorInvalidValue;
+  if (k < 0) k = 0;
+  const int tn = ceil_div(n, GBN), tm = ceil_div(m, GBM);
+  dim3 grid(tm * tn, grid_batch(batch), outer);
+  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao, bo, co);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
@@ -194,14 +212,8 @@ hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T
       hipLaunchKernelGGL((k_gemv<T, 8>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
     return hipGetLastError();
   }
-  if (k <= 0) {
-    // C = beta * C: run the kernel with a zero-length K loop.
-    k = 0;
-  }
-  const int tn = ceil_div(n, GBN), tm = ceil_div(m, GBM);
-  dim3 grid(tm * tn, grid_batch(batch));
-  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn);
-  return hipGetLastError();
+  // k <= 0: C = beta * C through a zero-length K loop
+  return gemm_outer<T>(1, 0, 0, 0, batch, m, n, k, alpha, A, B, beta, C, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -872,6 +884,152 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
   return laswp<T>(batch, nrhs, 0, n, false, B, ipiv, pbs, s);
 }
 
+// ---------------------------------------------------------------------------
+// Inverse from getrf factors: A^{-1} = U^{-1} L^{-1} P.
+//
+// Both triangles are inverted as lower-triangular matrices (U through its
+// transpose), in a workspace padded to np = 32 * 2^l >= n with the identity:
+//   W[z]  z <  nb: LU of matrix z (its strict lower part is L, unit diagonal)
+//         z >= nb: (LU of matrix z - nb)^T (its lower part is U^T)
+//   X[z]  the inverse of that lower-triangular matrix.
+// The 32 x 32 diagonal blocks are inverted by substitution in one launch;
+// then recursive doubling joins neighbouring blocks, for s = 32, 64, ...:
+//   X21 = -X22 (W21 X11)
+// for every pair of s-blocks of every matrix in two batched GEMMs, so the
+// whole inversion is 2 log2(np / 32) + 2 launches, without a sequential
+// block loop.  Finally Y = X[nb + b]^T X[b] = U^{-1} L^{-1} and the column
+// permutation A^{-1}[:, j] = Y[:, iperm[j]] (iperm[v] = the row where the
+// interchanges send row v).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_getri_prep(int n, int np, int nb, View<T> LU, T* W) {
+  __shared__ T t[32][33];
+  const int b = blockIdx.z, r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t nn = (int64_t)np * np;
+  T* W0 = W + b * nn;
+  T* W1 = W + (nb + b) * nn;
+  const int nm1 = n - 1;
+#pragma unroll
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    const T v = *LU.at(b, r < nm1 ? r : nm1, c < nm1 ? c : nm1);
+    const T w = (r < n && c < n) ? v : T(0);
+    t[i][tx] = w;
+    W0[(int64_t)r * np + c] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = ty; i < 32; i += 8) W1[(int64_t)(c0 + i) * np + r0 + tx] = t[tx][i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_getri_diag(int n, int np, int nb, const T* W, T* X) {
+  __shared__ T Ls[32][33];
+  const int z = blockIdx.y, d0 = blockIdx.x * 32, tid = threadIdx.x;
+  const bool unit = z < nb;
+  const int64_t nn = (int64_t)np * np;
+  const T* A = W + z * nn;
+  T* Xz = X + z * nn;
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    const T v = A[(int64_t)(d0 + r) * np + d0 + c];
+    Ls[r][c] = (d0 + r < n && d0 + c < n) ? v : ((r == c) ? T(1) : T(0));
+  }
+  __syncthreads();
+  if (tid < 32) {
+    // column c of the block inverse: forward substitution on e_c
+    const int c = tid;
+    T x[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      T v = (i == c) ? T(1) : T(0);
+#pragma unroll
+      for (int j = 0; j < i; ++j) v -= Ls[i][j] * x[j];
+      x[i] = unit ? v : v / Ls[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) Xz[(int64_t)(d0 + i) * np + d0 + c] = x[i];
+  }
+  // the strip right of the diagonal block is zero in a lower-triangular inverse
+  for (int r = 0; r < 32; ++r)
+    for (int c = d0 + 32 + tid; c < np; c += 256) Xz[(int64_t)(d0 + r) * np + c] = T(0);
+}
+
+__global__ __launch_bounds__(1024) void k_piv_to_iperm(int n, const int* ipiv, int64_t pbs, int* iperm, int64_t qbs) {
+  extern __shared__ int P[];
+  const int b = blockIdx.x;
+  for (int r = threadIdx.x; r < n; r += 1024) P[r] = ipiv[b * pbs + r] - 1;
+  __syncthreads();
+  for (int v = threadIdx.x; v < n; v += 1024) {
+    int pos = v;
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) {
+      const int pk = P[k];
+      pos = (pos == k) ? pk : ((pos == pk) ? k : pos);
+    }
+    iperm[b * qbs + v] = pos;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_getri_gather(int n, int np, const T* Y, const int* iperm, View<T> Xo) {
+  const int b = blockIdx.y, i = blockIdx.x;
+  const T* y = Y + (int64_t)b * np * np + (int64_t)i * np;
+  const int* q = iperm + (int64_t)b * n;
+  for (int j = threadIdx.x; j < n; j += 256) *Xo.at(b, i, j) = y[q[j]];
+}
+
+size_t getri_ws_bytes(int batch, int n, size_t elem) {
+  int np = 32;
+  while (np < n) np *= 2;
+  const size_t nn = (size_t)np * np;
+  // W and X (2 * batch each), the pair products (2 * batch quarter matrices), iperm
+  return elem * (4 * (size_t)batch * nn + 2 * (size_t)batch * (nn / 4)) + sizeof(int) * (size_t)batch * n + 256;
+}
+
+template <typename T>
+hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> Xo, void* ws, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (batch > 32767 || n > 16384) return hipErrorInvalidValue;
+  int np = 32;
+  while (np < n) np *= 2;
+  const int64_t nn = (int64_t)np * np;
+  T* W = reinterpret_cast<T*>(ws);
+  T* X = W + 2 * batch * nn;
+  T* Tm = X + 2 * batch * nn;
+  int* iperm = reinterpret_cast<int*>(Tm + 2 * batch * (nn / 4));
+  const int z = 2 * batch, nt = np / 32;
+  hipError_t e;
+  hipLaunchKernelGGL(k_getri_prep<T>, dim3(nt, nt, batch), dim3(256), 0, s, n, np, batch, LU, W);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_getri_diag<T>, dim3(nt, z), dim3(256), 0, s, n, np, batch, (const T*)W, X);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_piv_to_iperm, dim3(batch), dim3(1024), sizeof(int) * n, s, n, ipiv, pbs, iperm,
+                     (int64_t)n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  for (int sb = 32; sb < np; sb *= 2) {
+    // pairs of s-blocks: pair j has its first block at (2 j s, 2 j s)
+    const int npairs = np / (2 * sb);
+    const int64_t ps = 2LL * sb * (np + 1), ts = (int64_t)sb * sb;
+    const View<T> W21{W + (int64_t)sb * np, np, 1, ps};
+    const View<T> X11{X, np, 1, ps};
+    const View<T> X21{X + (int64_t)sb * np, np, 1, ps};
+    const View<T> X22{X + (int64_t)sb * np + sb, np, 1, ps};
+    const View<T> Tv{Tm, sb, 1, ts};
+    e = gemm_outer<T>(z, nn, nn, nn / 4, npairs, sb, sb, sb, T(1), W21, X11, T(0), Tv, s);
+    if (e != hipSuccess) return e;
+    e = gemm_outer<T>(z, nn, nn / 4, nn, npairs, sb, sb, sb, T(-1), X22, Tv, T(0), X21, s);
+    if (e != hipSuccess) return e;
+  }
+  // Y = U^{-1} L^{-1} into W[0..batch) (the copies of LU are no longer read)
+  e = gemm<T>(batch, n, n, n, T(1), View<T>{X + batch * nn, 1, np, nn}, View<T>{X, np, 1, nn}, T(0),
+              View<T>{W, np, 1, nn}, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_getri_gather<T>, dim3(n, batch), dim3(256), 0, s, n, np, (const T*)W, (const int*)iperm, Xo);
+  return hipGetLastError();
+}
+
 #define SLAB_INSTANTIATE(T)                                                                                        \
   template hipError_t gemm<T>(int, int, int, int, T, View<T>, View<T>, T, View<T>, hipStream_t);                  \
   template hipError_t trsm_left<T>(int, int, int, bool, bool, View<T>, View<T>, hipStream_t);                     \
@@ -879,7 +1037,8 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
   template hipError_t getrf<T>(int, int, int, int, int, View<T>, int*, int64_t, int*, hipStream_t);               \
   template hipError_t getrs<T>(int, int, int, bool, View<T>, const int*, int64_t, View<T>, hipStream_t);         \
   template hipError_t copy_batched<T>(int, int, int, View<T>, View<T>, hipStream_t);                              \
-  template hipError_t set_identity<T>(int, int, View<T>, hipStream_t);
+  template hipError_t set_identity<T>(int, int, View<T>, hipStream_t);                                            \
+  template hipError_t getri<T>(int, int, View<T>, const int*, int64_t, View<T>, void*, hipStream_t);
 SLAB_INSTANTIATE(double)
 SLAB_INSTANTIATE(float)
 

@@ -75,6 +75,11 @@ hipError_t copy_batched(int batch, int rows, int cols, View<T> src, View<T> dst,
 // dst(b) = I (n x n)
 template <typename T>
 hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
+// X(b) = A(b)^{-1} from the factors of getrf (n x n), out of place; ws holds
+// getri_ws_bytes(batch, n, sizeof(T)) bytes
+size_t getri_ws_bytes(int batch, int n, size_t elem);
+template <typename T>
+hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> X, void* ws, hipStream_t s);
 // getrs for nrhs <= 8, non-transposed, one workgroup per matrix
 // LU solve with the composed pivot permutation (perm from piv_to_perm_batched),
 // nrhs <= 8, one wavefront per matrix

@@ -36,6 +36,8 @@ def _load():
         "slab_getrf": ([_int, _int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _vp], _int),
         "slab_getrs": ([_int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_piv_to_perm": ([_int, _int, _vp, _i64, _vp, _i64, _vp], _int),
+        "slab_getri_ws": ([_int, _int, _int], ctypes.c_size_t),
+        "slab_getri": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp], _int),
         "slab_getrs_perm": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_btri_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                               _i64, _vp, _vp], _int),
@@ -182,6 +184,22 @@ def piv_to_perm(ipiv, n=None):
     _check(LIB.slab_piv_to_perm(b, n, _p(ipiv), ipiv.stride(0), _p(perm), perm.stride(0), _stream()),
            "slab_piv_to_perm")
     return perm
+
+
+def getri(LU, ipiv, X=None):
+    """Batched inverse from getrf factors: X (batch, n, n) = A^{-1}, out of
+    place (a new tensor unless X is given)."""
+    _require(LU.dim() == 3 and LU.stride(2) == 1 and LU.shape[1] == LU.shape[2], "getri expects square row-major LU")
+    b, n, _ = LU.shape
+    _require(ipiv.dtype == torch.int32 and ipiv.shape[0] == b and ipiv.shape[1] >= n and ipiv.stride(-1) == 1,
+             "ipiv shape/dtype")
+    if X is None:
+        X = torch.empty(b, n, n, dtype=LU.dtype, device=LU.device)
+    _require(X.shape == LU.shape and X.dtype == LU.dtype and X.stride(2) == 1, "getri output layout")
+    ws = torch.empty(int(LIB.slab_getri_ws(_dt(LU), b, n)), dtype=torch.uint8, device=LU.device)
+    _check(LIB.slab_getri(_dt(LU), b, n, _p(LU), LU.stride(1), LU.stride(0), _p(ipiv), ipiv.stride(0), _p(X),
+                          X.stride(1), X.stride(0), _p(ws), _stream()), "slab_getri")
+    return X
 
 
 def getrs(LU, ipiv, B, trans=False, perm=None):

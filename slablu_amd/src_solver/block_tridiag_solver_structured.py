@@ -36,19 +36,14 @@ def _dense(x, device):
 
 
 def _inverse(S):
-    """S (b, n, n) -> S^{-1} (b, n, n) = U^{-1} L^{-1} P from getrf; P (the
-    row interchanges applied to the identity) is written directly from the
-    composed permutation, then the two triangular solves run on it."""
+    """S (b, n, n) -> S^{-1} (b, n, n) = U^{-1} L^{-1} P from getrf (S is
+    overwritten by its factors); getri inverts both triangles by recursive
+    doubling, so no step runs down the diagonal block by block."""
     b, n, _ = S.shape
     ipiv = torch.empty(b, n, dtype=torch.int32, device=S.device)
     info = torch.zeros(b, dtype=torch.int32, device=S.device)
     hip.getrf(S, ipiv, info=info)
-    perm = hip.piv_to_perm(ipiv)
-    X = torch.zeros(b, n, n, dtype=S.dtype, device=S.device)
-    X.scatter_(2, perm.long().unsqueeze(-1), 1.0)            # row r of P is e_{perm[r]}
-    hip.trsm(S, X, lower=True, unit=True)
-    hip.trsm(S, X, lower=False, unit=False)
-    return X, info
+    return hip.getri(S, ipiv), info
 
 
 class HBS_BlockTridiagonal:

@@ -9,14 +9,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "slab_hip.h")).read()
-    return sorted(set(re.findall(r"^int\s+(slab_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|size_t)\s+(slab_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_entry_points():
     names = _declared()
     for must in ("slab_gemm", "slab_getrf", "slab_btri_factor", "slab_btri_solve", "slab_btri_inv_factor",
                  "slab_btri_inv_solve", "slab_leaf_assemble", "slab_leaf_assemble_nodes", "slab_leaf_residual", "slab_btri_factor_twisted", "slab_btri_solve_twisted",
-                 "slab_btri_fused"):
+                 "slab_btri_fused", "slab_getri", "slab_getri_ws"):
         assert must in names
 
 
@@ -54,3 +54,14 @@ def test_bad_arguments_are_rejected_without_a_gpu():
         [ctypes.c_void_p(0), ctypes.c_int64(0), ctypes.c_int64(0)] * 3 + \
         [ctypes.c_void_p(0), ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_void_p(0), ctypes.c_void_p(0)]
     assert lib.slab_btri_factor_twisted(*targs) == -1
+    # getri workspace: W and X (2 batch padded matrices each), pair products,
+    # pivots; a missing workspace is rejected
+    lib.slab_getri_ws.restype = ctypes.c_size_t
+    np_, b = 1024, 2
+    assert lib.slab_getri_ws(0, b, 1000) == 8 * (4 * b * np_ * np_ + 2 * b * np_ * np_ // 4) + 4 * b * 1000 + 256
+    assert lib.slab_getri_ws(5, b, 1000) == 0
+    lib.slab_getri.restype = ctypes.c_int
+    gargs = [ctypes.c_int(0), ctypes.c_int(1), ctypes.c_int(8), ctypes.c_void_p(0), ctypes.c_int64(8),
+             ctypes.c_int64(64), ctypes.c_void_p(0), ctypes.c_int64(8), ctypes.c_void_p(0), ctypes.c_int64(8),
+             ctypes.c_int64(64), ctypes.c_void_p(0), ctypes.c_void_p(0)]
+    assert lib.slab_getri(*gargs) == -1

@@ -163,6 +163,43 @@ def test_getrs(dtype, trans):
     assert _rel(X, ref) < 100 * TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 257, 1000])
+def test_getri_inverse(dtype, n):
+    """getrf + getri against LAPACK's inverse; the residual A X - I relative
+    to |A| |X| is independent of the conditioning of the random matrices."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(n)
+    b = 2 if n >= 257 else 3
+    A0 = torch.randn(b, n, n, generator=g, dtype=torch.float64)
+    A = A0.to("cuda", dtype).contiguous()
+    ipiv = torch.zeros(b, n, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv)
+    X = hip.getri(A, ipiv).double().cpu()
+    eye = torch.eye(n, dtype=torch.float64)
+    eps = torch.finfo(dtype).eps
+    for i in range(b):
+        res = torch.linalg.norm(A0[i] @ X[i] - eye) / (torch.linalg.norm(A0[i]) * torch.linalg.norm(X[i]))
+        assert res < 10 * n ** 0.5 * eps, (i, res.item())
+        ref = torch.linalg.inv(A0[i])
+        cond = torch.linalg.cond(A0[i]).item()
+        assert _rel(X[i], ref) < 20 * cond * eps, (i, _rel(X[i], ref), cond)
+
+
+@pytest.mark.parametrize("n", [7, 64, 1000])
+def test_getri_permutation_is_exact(n):
+    """A permutation matrix factors as P^T I I; its inverse is its transpose,
+    bit for bit, so any slip in the column interchanges shows."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(3 + n)
+    P = torch.stack([torch.eye(n, dtype=torch.float64)[torch.randperm(n, generator=g)] for _ in range(2)])
+    A = P.to("cuda").contiguous()
+    ipiv = torch.zeros(2, n, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv)
+    X = hip.getri(A, ipiv).cpu()
+    assert torch.equal(X, P.transpose(1, 2))
+
+
 def _btri_dense(diag, sub, sup):
     b, m, bs, _ = diag.shape
     M = torch.zeros(b, m * bs, m * bs, dtype=torch.float64)
