@@ -46,14 +46,10 @@ static inline int grid_batch(int batch) { return batch < 65535 ? batch : 65535; 
 // ---------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
-template <typename T>
+template <typename T, typename I>
 __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
                                               T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co) {
   typedef typename Mfma<T>::acc_t acc_t;
-  // second batch level (grid z): whole-operand offsets
-  A.p += blockIdx.z * ao;
-  B.p += blockIdx.z * bo;
-  C.p += blockIdx.z * co;
   __shared__ T As[GBK][GBM + 1];
   __shared__ T Bs[GBK][GBN + 1];
   const int tile = blockIdx.x;
@@ -64,29 +60,58 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
   const bool b_rowmajor = (B.cs == 1);
   const int mm1 = m - 1, nm1 = n - 1, km1 = k - 1;
   constexpr int NU = GBM * GBK / 256;
+  // element offsets inside one matrix in I (32-bit whenever the host has
+  // checked that they fit); row / column parts fixed per thread, so only the
+  // K index moves between chunks.  Rows of A beyond m and columns of B
+  // beyond n are read clamped and feed only C entries that are never
+  // stored; K beyond k is zeroed.
+  I a_off[NU], b_off[NU];
+  int a_k[NU], b_k[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int e = tid + 256 * u;
+    int r, kk, c, kb;
+    if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
+    if (b_rowmajor) { kb = e / GBN; c = e % GBN; } else { kb = e % GBK; c = e / GBK; }
+    a_off[u] = (I)(m0 + r < mm1 ? m0 + r : mm1) * (I)A.rs;
+    b_off[u] = (I)(n0 + c < nm1 ? n0 + c : nm1) * (I)B.cs;
+    a_k[u] = kk;
+    b_k[u] = kb;
+  }
+  I c_row[2][4], c_col[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
+      c_row[i][q] = (I)(row < mm1 ? row : mm1) * (I)C.rs;
+    }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+    c_col[j] = (I)(col < nm1 ? col : nm1) * (I)C.cs;
+  }
   for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    // second batch level (grid z): whole-operand offsets
+    const T* Ab = A.p + blockIdx.z * ao + b * A.bs;
+    const T* Bb = B.p + blockIdx.z * bo + b * B.bs;
+    T* Cb = C.p + blockIdx.z * co + b * C.bs;
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     // operands of the next K chunk travel in registers while the current
-    // chunk is multiplied; out-of-range entries are read clamped and zeroed
+    // chunk is multiplied
     T ra[NU], rb[NU];
     auto load = [&](int k0) {
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-        const int e = tid + 256 * u;
-        int r, kk;
-        if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
-        const int gr = m0 + r, gk = k0 + kk;
-        const T va = *A.at(b, gr < mm1 ? gr : mm1, gk < km1 ? gk : km1);
-        ra[u] = (gr < m && gk < k) ? va : T(0);
-        int c, kb;
-        if (b_rowmajor) { kb = e / GBN; c = e % GBN; } else { kb = e % GBK; c = e / GBK; }
-        const int gc = n0 + c, gkb = k0 + kb;
-        const T vb = *B.at(b, gkb < km1 ? gkb : km1, gc < nm1 ? gc : nm1);
-        rb[u] = (gc < n && gkb < k) ? vb : T(0);
+        const int gk = k0 + a_k[u], gkb = k0 + b_k[u];
+        const T va = Ab[a_off[u] + (I)(gk < km1 ? gk : km1) * (I)A.cs];
+        ra[u] = (gk < k) ? va : T(0);
+        const T vb = Bb[b_off[u] + (I)(gkb < km1 ? gkb : km1) * (I)B.rs];
+        rb[u] = (gkb < k) ? vb : T(0);
       }
     };
     if (k > 0) load(0);
@@ -113,20 +138,6 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
       }
       __syncthreads();
     }
-    T cv[2][2][4];
-    if (beta != T(0)) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
-            cv[i][j][q] = *C.at(b, row < mm1 ? row : mm1, col < nm1 ? col : nm1);
-          }
-        }
-    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -137,8 +148,8 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
           const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
           if (row < m && col < n) {
             T v = alpha * acc[i][j][q];
-            if (beta != T(0)) v += beta * cv[i][j][q];
-            *C.at(b, row, col) = v;
+            if (beta != T(0)) v += beta * Cb[c_row[i][q] + c_col[j]];
+            Cb[c_row[i][q] + c_col[j]] = v;
           }
         }
       }
@@ -194,7 +205,18 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   if (k < 0) k = 0;
   const int tn = ceil_div(n, GBN), tm = ceil_div(m, GBM);
   dim3 grid(tm * tn, grid_batch(batch), outer);
-  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao, bo, co);
+  // 32-bit offsets inside each matrix when every element offset fits
+  auto span = [](int rows, int cols, const View<T>& V) {
+    const int64_t r = rows > 0 ? rows - 1 : 0, c = cols > 0 ? cols - 1 : 0;
+    return r * (V.rs < 0 ? -V.rs : V.rs) + c * (V.cs < 0 ? -V.cs : V.cs);
+  };
+  const int64_t lim = INT_MAX;
+  if (span(m, k, A) < lim && span(k, n, B) < lim && span(m, n, C) < lim)
+    hipLaunchKernelGGL((k_gemm<T, int>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao, bo,
+                       co);
+  else
+    hipLaunchKernelGGL((k_gemm<T, int64_t>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
+                       bo, co);
   return hipGetLastError();
 }
 
