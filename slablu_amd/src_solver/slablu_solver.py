@@ -24,10 +24,11 @@ sweep, slab sweeps, leaf recovery (slablu_solver.py:489-666).
 
 precision='mixed' runs steps 1-4a in FP32 (interface LU in FP64) and
 refines the full discrete system (leaf collocation equations + flux
-continuity on the faces) with FP64 residuals until the change of the face
-solution falls below `refine_tol` (relative, default 1e-8: the change of a
-step bounds the error of the iterate before it, and the iterate returned is
-one contraction of the FP32 preconditioner, about 0.02, better).  The default refinement is
+continuity on the faces) with FP64 residuals until the estimated error of
+the face solution falls below `refine_tol` (relative, default 1e-8).  The
+estimate is the change c_k of the last step, or, once two corrections after
+the first step contract by rho = c_k / c_{k-1} < 0.1, the geometric tail
+c_k rho / (1 - rho) that the iterate still lacks.  The default refinement is
 flexible GMRES (FP64 Arnoldi on the un-condensed operator, right-
 preconditioned by the FP32 two-level solve); refine_method='richardson' runs
 plain iterative refinement instead (one FP32 solve per correction).
@@ -590,9 +591,21 @@ class Domain_Solver:
         rI, rF = self.full_residual(x, dI)
         return -rI, -rF
 
+    def _converged(self, c, prev):
+        """Stop test on the relative change c of the last step (prev: the
+        step before, or None): the error left after a geometrically
+        converging step is about c rho / (1 - rho), rho = c / prev, used
+        when rho < 0.1; otherwise c itself."""
+        if prev is not None and prev > 0.0:
+            rho = c / prev
+            if rho < 0.1:
+                c = c * rho / (1.0 - rho)
+        return c <= self.refine_tol
+
     def _refine_richardson(self, x_face, uc, fb=None):
         """x <- x + M^{-1}(b - A x) until the face correction is small."""
         Ic = self._Ic()
+        prev = None
         for it in range(self.refine_maxit):
             rI, rF = self.full_residual(x_face, uc, fb=fb)
             dI, dC = self._correction(rI, rF)
@@ -600,8 +613,9 @@ class Domain_Solver:
             uc += dI
             rel_corr = (torch.linalg.norm(dC) / torch.linalg.norm(x_face)).item()
             self.refine_history.append(rel_corr)
-            if rel_corr <= self.refine_tol:
+            if self._converged(rel_corr, prev):
                 break
+            prev = rel_corr if it >= 1 else None
 
     def _refine_fgmres(self, x_face, uc, fb=None):
         """Flexible GMRES on the full system, right-preconditioned by the FP32
@@ -610,8 +624,8 @@ class Domain_Solver:
         FP64 residual ||b - A x|| is minimised over x0 + span(Z_j) even though
         M is applied in FP32.  Stops when the change of the face solution
         between consecutive iterates is below refine_tol relative to the face
-        solution (the quantity plain refinement tests), restarting every
-        refine_restart steps."""
+        solution (the quantity plain refinement tests, with the same error
+        estimate), restarting every refine_restart steps."""
         hps, ops = self.hps, self.hps.ops
         dev = self.device
         Ic = self._Ic()
@@ -637,6 +651,7 @@ class Domain_Solver:
 
         steps = 0
         done = False
+        prev = None
         while steps < self.refine_maxit and not done:
             rI, rF = self.full_residual(x_face, uc, fb=fb)
             V = torch.empty(nc, kmax + 1, ntot, dtype=torch.float64, device=dev)
@@ -678,8 +693,11 @@ class Domain_Solver:
                 self.refine_history.append(rel)
                 y_prev = y
                 steps += 1
-                if rel <= self.refine_tol or steps >= self.refine_maxit:
-                    done = rel <= self.refine_tol
+                done = self._converged(rel, prev)
+                # the first step moves from the starting guess, not by a
+                # contraction: rates start from the second step
+                prev = rel if steps >= 2 else None
+                if done or steps >= self.refine_maxit:
                     break
             yd = torch.from_numpy(y_prev).to(dev)
             upd = torch.zeros(nc, ntot, dtype=torch.float64, device=dev)

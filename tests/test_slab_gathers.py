@@ -99,3 +99,18 @@ def test_row_couplings(n0, n1, p, kh, bufc):
             AXr = A[np.ix_(Ix, Ir)].copy()
             assert np.array_equal(dom.cXr[k].numpy(), Xr)
             assert np.array_equal(dom.cAXr[k].numpy(), AXr)
+
+
+def test_refinement_stop_test_uses_the_geometric_tail():
+    """The stop test on the relative face changes c_k: with a contraction
+    rho = c_k / c_{k-1} below 0.1 the error left is c_k rho / (1 - rho);
+    a slow or first step is judged by c_k itself."""
+    import types
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    s = types.SimpleNamespace(refine_tol=1e-8)
+    conv = lambda c, prev: Domain_Solver._converged(s, c, prev)
+    assert not conv(2.7e-7, None)                 # no history: c itself
+    assert conv(2.7e-7, 1.8e-5)                   # rho 0.015: tail 4.1e-9
+    assert not conv(2.7e-7, 1.0e-6)               # rho 0.27: too slow to trust the tail
+    assert not conv(5e-7, 2e-5)                   # rho 0.025: tail 1.28e-8 > tol
+    assert conv(9e-9, 1e-8)                       # slow but already below tol
