@@ -139,6 +139,28 @@ int slab_btri_solve_twisted(int dtype, int b, int m, int bs,
                             const int* ipiv, int64_t p_bstride, int64_t p_istride,
                             void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* stream);
 
+/* Inverse triangular factors of every diagonal block after
+ * slab_btri_factor (or _twisted): diag's packed LU is replaced by U^{-1}
+ * and linv (same layout) receives L^{-1} P, so the sweeps of
+ * slab_btri_solve_twisted_inv are block products on the matrix cores
+ * instead of triangular substitutions.  The Gaussian elimination and its
+ * pivots are those of block_tridiag_solver_dense.py:152-177; only the way
+ * each diagonal block is applied differs.  Blocks must be contiguous
+ * (d_istride = bs*bs, d_bstride = m*bs*bs, p_istride = bs, p_bstride =
+ * m*bs); ws: slab_getri_ws(dtype, b*m, bs) bytes. */
+int slab_btri_invert_diag(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride,
+                          const int* ipiv, int64_t p_bstride, int64_t p_istride, void* linv, void* ws,
+                          void* stream);
+
+/* slab_btri_solve_twisted with the factors of slab_btri_invert_diag
+ * (lusolve_sweep, src_solver/block_tridiag_solver_dense.py:179-202, with
+ * the permutation folded into L^{-1} P). */
+int slab_btri_solve_twisted_inv(int dtype, int b, int m, int bs,
+                                const void* diag, int64_t d_bstride, int64_t d_istride, const void* linv,
+                                const void* sub, int64_t l_bstride, int64_t l_istride,
+                                const void* sup, int64_t u_bstride, int64_t u_istride,
+                                void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* stream);
+
 /* 1 when blocks of size bs run in the fused one-workgroup-per-chain LU-form
  * kernels (the diagonal block, a right-hand-side chunk and staging fit in
  * LDS), 0 otherwise, -1 for a bad dtype. */

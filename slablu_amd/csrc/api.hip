@@ -256,6 +256,42 @@ int slab_btri_solve_twisted(int dtype, int b, int m, int bs, const void* diag, i
                                              p_istride, (float*)f, f_bstride, ldf, nrhs, S(stream)));
 }
 
+int slab_btri_invert_diag(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride,
+                          const int* ipiv, int64_t p_bstride, int64_t p_istride, void* linv, void* ws,
+                          void* stream) {
+  if (b < 0 || m < 1 || bs < 1) return -1;
+  // all blocks must form one batch of contiguous bs x bs blocks
+  if (d_istride != (int64_t)bs * bs || (b > 1 && d_bstride != (int64_t)m * bs * bs)) return -1;
+  if (p_istride != bs || (b > 1 && p_bstride != (int64_t)m * bs)) return -1;
+  if (b == 0) return 0;
+  if (linv == nullptr || ws == nullptr || (int64_t)b * m > 32767) return -1;
+  const int nbk = b * m;
+  const int64_t bb = (int64_t)bs * bs;
+  if (dtype == SLAB_F64)
+    return status(getri_factors<double>(nbk, bs, view<double>(diag, bs, 1, bb), ipiv, bs, view<double>(linv, bs, 1, bb),
+                                        view<double>(diag, bs, 1, bb), ws, S(stream)));
+  if (dtype == SLAB_F32)
+    return status(getri_factors<float>(nbk, bs, view<float>(diag, bs, 1, bb), ipiv, bs, view<float>(linv, bs, 1, bb),
+                                       view<float>(diag, bs, 1, bb), ws, S(stream)));
+  return -1;
+}
+
+int slab_btri_solve_twisted_inv(int dtype, int b, int m, int bs, const void* diag, int64_t d_bstride,
+                                int64_t d_istride, const void* linv, const void* sub, int64_t l_bstride,
+                                int64_t l_istride, const void* sup, int64_t u_bstride, int64_t u_istride, void* f,
+                                int64_t f_bstride, int64_t ldf, int nrhs, void* stream) {
+  if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs || linv == nullptr || slab_btri_fused(dtype, bs) != 1)
+    return -1;
+  if (nrhs == 0 || b == 0) return 0;
+  if (dtype == SLAB_F64)
+    return status(btri_lu_solve_twisted<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub,
+                                                l_bstride, l_istride, (double*)sup, u_bstride, u_istride, nullptr, 0,
+                                                0, (double*)f, f_bstride, ldf, nrhs, S(stream), (const double*)linv));
+  return status(btri_lu_solve_twisted<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
+                                             l_istride, (float*)sup, u_bstride, u_istride, nullptr, 0, 0, (float*)f,
+                                             f_bstride, ldf, nrhs, S(stream), (const float*)linv));
+}
+
 int slab_btri_inv_fused(int dtype, int bs) {
   if (dtype == SLAB_F64) return btri_inv_fused_ok<double>(bs) ? 1 : 0;
   if (dtype == SLAB_F32) return btri_inv_fused_ok<float>(bs) ? 1 : 0;

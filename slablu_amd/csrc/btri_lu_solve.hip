@@ -68,8 +68,11 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
       i = m - 1 - (t >> 1);
       return 2;
     };
+    // inverse factors: DIAGL multiplies by L^{-1} P (linv), DIAGU by U^{-1}
+    // (diag), both plain block products
+    const bool inv = a.linv != nullptr;
     // the last block's DIAGU of a full sweep reuses the resident DIAGL block
-    auto resident = [&](int k, int i) -> bool { return mode == 0 && k == 2 && i == m - 1; };
+    auto resident = [&](int k, int i) -> bool { return !inv && mode == 0 && k == 2 && i == m - 1; };
     T* fb = a.f + b * a.fbs;
     const int* pb = a.ipiv + b * a.pbs;
     T* Xc = B0;
@@ -78,6 +81,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
     auto block_src = [&](int k, int i) -> const T* {
       if (k == 1) return a.sub + b * a.lbs + (int64_t)i * a.lis;
       if (k == 3) return a.sup + b * a.ubs + (int64_t)i * a.uis;
+      if (k == 0 && inv) return a.linv + b * a.dbs + (int64_t)i * a.dis;
       return a.diag + b * a.dbs + (int64_t)i * a.dis;
     };
     // forward sweeps start at the first block whose right-hand side tile
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
           pfx[u] = (e < nf && c < ncols) ? v : T(0);
         }
       }
-      if (k == 0) {
+      if (k == 0 && !inv) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int r = tid + 256 * u;
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
           if (e < nf) Xc[r * ldx + c] = pfx[u];
         }
       }
-      if (k == 0) {
+      if (k == 0 && !inv) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int r = tid + 256 * u;
@@ -181,24 +185,40 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
       if (s + 1 < nst) prefetch(s + 1);
       int i;
       const int k = decode(s, i);
+      // Out = Mt In (Out may be In) for the inverse-factor stages
+      auto block_set = [&](const T* In, T* Out) {
+        if constexpr (NR == 1)
+          sub_mat_vec1<T, true>(Mt, In, Out, bs, W);
+        else if constexpr (NR == 32)
+          sub_mat_mfma32<T, true>(Mt, In, Out, bs, ldx);
+        else
+          sub_mat_vec<T, RPT, true>(Mt, In, Out, bs, nr, ldx);
+      };
       if (k == 0) {
-        piv_to_perm(P, bs, Q);
-        for (int e = tid; e < nf; e += 256) {
-          const int r = e >> lgnr, c = e & (nr - 1);
-          Y[r * ldx + c] = Xc[Q[r] * ldx + c];
+        if (inv) {
+          block_set(Xc, Y);
+        } else {
+          piv_to_perm(P, bs, Q);
+          for (int e = tid; e < nf; e += 256) {
+            const int r = e >> lgnr, c = e & (nr - 1);
+            Y[r * ldx + c] = Xc[Q[r] * ldx + c];
+          }
+          __syncthreads();
+          tri_solve_n<T, false, true, NR>(Mat, bs, Y, ldx);
         }
-        __syncthreads();
-        tri_solve_n<T, false, true, NR>(Mat, bs, Y, ldx);
         store_f(i, Y);
       } else if (k == 1 || k == 3) {
         if constexpr (NR == 1)
           sub_mat_vec1<T>(Mt, Y, Xc, bs, W);
+        else if constexpr (NR == 32)
+          sub_mat_mfma32<T>(Mt, Y, Xc, bs, ldx);
         else
           sub_mat_vec<T, RPT>(Mt, Y, Xc, bs, nr, ldx);
       } else {
         const bool own = mode == 0 && i == m - 1;
         T* Z = own ? Y : Xc;
-        tri_solve_n<T, true, false, NR>(Mat, bs, Z, ldx);
+        if (inv) block_set(Z, Z);
+        else tri_solve_n<T, true, false, NR>(Mat, bs, Z, ldx);
         store_f(i, Z);
         if (!own) {
           T* t = Xc;
@@ -272,14 +292,15 @@ hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int6
 template <typename T>
 hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                  int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
-                                 int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s) {
+                                 int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s,
+                                 const T* linv) {
   const int k = m / 2, lb = m - 1 - k;
   int* iv = const_cast<int*>(ipiv);
   const int64_t fis = (int64_t)bs * ldf;
   ChainSet<T> none{};
-  ChainSet<T> top{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv, pbs, pis, nullptr, f, fbs, fis, k, nb, 1};
+  ChainSet<T> top{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv, pbs, pis, nullptr, f, fbs, fis, k, nb, 1, linv};
   ChainSet<T> bot = mirrored<T>(nb, m, lb, bs, diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv, pbs, pis, nullptr,
-                                f, fbs, ldf);
+                                f, fbs, ldf, linv);
   hipError_t e;
   if (k > 0) {
     if ((e = solve_sets<T>(bs, top, bot, ldf, nrhs, 1, s)) != hipSuccess) return e;
@@ -296,7 +317,7 @@ hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, in
     }
   }
   ChainSet<T> mid{diag + (int64_t)k * dis, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv + (int64_t)k * pis, pbs, pis,
-                  nullptr, f + (int64_t)k * fis, fbs, fis, 1, nb, 0};
+                  nullptr, f + (int64_t)k * fis, fbs, fis, 1, nb, 0, linv != nullptr ? linv + (int64_t)k * dis : nullptr};
   if ((e = solve_sets<T>(bs, mid, none, ldf, nrhs, 0, s)) != hipSuccess) return e;
   if (k > 0) return solve_sets<T>(bs, top, bot, ldf, nrhs, 2, s);
   return hipSuccess;
@@ -308,7 +329,7 @@ hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, in
                                              int, hipStream_t);                                                    \
   template hipError_t btri_lu_solve_twisted<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,     \
                                                int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t,        \
-                                               int64_t, int, hipStream_t);
+                                               int64_t, int, hipStream_t, const T*);
 SLAB_BTRI_LU_SOLVE_INST(double)
 SLAB_BTRI_LU_SOLVE_INST(float)
 

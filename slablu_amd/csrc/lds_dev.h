@@ -40,6 +40,9 @@ struct ChainSet {
   int* info;
   T* f; int64_t fbs, fis;
   int m, nchains, open;
+  // inverse-factor sweeps when set: L^{-1} P of each diagonal block (diag's
+  // strides), with diag holding U^{-1}
+  const T* linv;
 };
 
 // The chain set that walks blocks m-1 .. m-len of each chain backwards:
@@ -47,9 +50,10 @@ struct ChainSet {
 template <typename T>
 inline ChainSet<T> mirrored(int nb, int m, int len, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                      int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs, int64_t pis, int* info,
-                     T* f, int64_t fbs, int64_t ldf) {
+                     T* f, int64_t fbs, int64_t ldf, const T* linv = nullptr) {
   ChainSet<T> c{};
   c.diag = diag + (int64_t)(m - 1) * dis; c.dbs = dbs; c.dis = -dis;
+  c.linv = linv != nullptr ? linv + (int64_t)(m - 1) * dis : nullptr;
   c.sub = sup + (int64_t)(m - 2) * uis; c.lbs = ubs; c.lis = -uis;
   c.sup = sub + (int64_t)(m - 2) * lis; c.ubs = lbs; c.uis = -lis;
   c.ipiv = ipiv + (int64_t)(m - 1) * pis; c.pbs = pbs; c.pis = -pis;
@@ -634,7 +638,9 @@ __device__ inline void stage_block(T* Mt, const T* src, int bs, float rbs, int l
 // Xc[r][c] -= sum_k Mt[r][k] Y[k][c] (all operands in LDS, Mt row stride
 // bs+1, X row stride ldx); thread (tr, tc) owns column tc and rows
 // tr + (256/nr) x, read from clamped rows so the k loop has no tests.
-template <typename T, int RPT>
+// SET: Xc = Mt Y instead of Xc -= Mt Y, with every read of Y done before
+// the first write (Xc may be Y).
+template <typename T, int RPT, bool SET = false>
 __device__ inline void sub_mat_vec(const T* Mt, const T* Y, T* Xc, int bs, int nr, int ldx) {
   const int tid = threadIdx.x;
   const int tc = tid % nr, tr = tid / nr, RG = 256 / nr;
@@ -654,17 +660,76 @@ __device__ inline void sub_mat_vec(const T* Mt, const T* Y, T* Xc, int bs, int n
 #pragma unroll
     for (int x = 0; x < RPT; ++x) acc[x] += Mt[off[x] + k] * y;
   }
+  if constexpr (SET) __syncthreads();
 #pragma unroll
   for (int x = 0; x < RPT; ++x) {
     const int r = tr + RG * x;
-    if (r < bs) Xc[r * ldx + tc] -= acc[x];
+    if (r < bs) {
+      if constexpr (SET) Xc[r * ldx + tc] = acc[x];
+      else Xc[r * ldx + tc] -= acc[x];
+    }
+  }
+  __syncthreads();
+}
+
+// Xc -= Mt Y (SET: Xc = Mt Y, Xc may be Y) for 32 columns on the matrix
+// cores (16 x 16 x 4 steps):
+// wavefront w owns column tile w & 1 and row tiles (w >> 1) + 2 j, so the
+// Y fragment of a K step is read once for all of its row tiles.  Rows of Mt
+// past bs are read clamped and feed only rows that are never written; K
+// past bs is zeroed.  bs <= 160.
+template <typename T, bool SET = false>
+__device__ inline void sub_mat_mfma32(const T* Mt, const T* Y, T* Xc, int bs, int ldx) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int MAXT = 5;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ld = bs + 1, bm1 = bs - 1;
+  const int ct = wave & 1, rt0 = wave >> 1, nrt = (bs + 15) >> 4;
+  const int cr = lane & 15, kq = lane >> 4;
+  const int ycol = ct * 16 + cr;
+  acc_t acc[MAXT];
+  int aoff[MAXT];
+#pragma unroll
+  for (int j = 0; j < MAXT; ++j) {
+    acc[j] = acc_t{0, 0, 0, 0};
+    const int r = (rt0 + 2 * j) * 16 + cr;
+    aoff[j] = (r < bm1 ? r : bm1) * ld;
+  }
+  for (int k0 = 0; k0 < bs; k0 += 4) {
+    const int k = k0 + kq;
+    const bool kin = k < bs;
+    const int kc = kin ? k : bm1;
+    const T yv = Y[kc * ldx + ycol];
+    const T b = kin ? yv : T(0);
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+      if (rt0 + 2 * j < nrt) {  // uniform in the wavefront
+        const T av = Mt[aoff[j] + kc];
+        acc[j] = Mfma<T>::mma(kin ? av : T(0), b, acc[j]);
+      }
+    }
+  }
+  if constexpr (SET) __syncthreads();
+#pragma unroll
+  for (int j = 0; j < MAXT; ++j) {
+    const int rt = rt0 + 2 * j;
+    if (rt < nrt) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rt * 16 + Mfma<T>::row(lane, q);
+        if (r < bs) {
+          if constexpr (SET) Xc[r * ldx + ycol] = acc[j][q];
+          else Xc[r * ldx + ycol] -= acc[j][q];
+        }
+      }
+    }
   }
   __syncthreads();
 }
 
 // Single-column Xc -= Mt y: for bs <= 128 two threads per row split k (the
 // upper half's partial sums pass through W, bs elements of LDS).
-template <typename T>
+template <typename T, bool SET = false>
 __device__ inline void sub_mat_vec1(const T* Mt, const T* Y, T* Xc, int bs, T* W) {
   const int tid = threadIdx.x, ld = bs + 1;
   if (bs <= 128) {
@@ -677,13 +742,22 @@ __device__ inline void sub_mat_vec1(const T* Mt, const T* Y, T* Xc, int bs, T* W
     for (int k = k0; k < k1; ++k) acc += Mt[rc + k] * Y[k];
     if (h == 1 && r < bs) W[r] = acc;
     __syncthreads();
-    if (h == 0 && r < bs) Xc[r] -= acc + W[r];
+    if (h == 0 && r < bs) {
+      if constexpr (SET) Xc[r] = acc + W[r];
+      else Xc[r] -= acc + W[r];
+    }
   } else {
-    for (int r = tid; r < bs; r += 256) {
-      T acc = T(0);
+    // bs <= 256: one row per thread
+    const int r = tid;
+    T acc = T(0);
+    if (r < bs) {
 #pragma unroll 8
       for (int k = 0; k < bs; ++k) acc += Mt[r * ld + k] * Y[k];
-      Xc[r] -= acc;
+    }
+    if constexpr (SET) __syncthreads();
+    if (r < bs) {
+      if constexpr (SET) Xc[r] = acc;
+      else Xc[r] -= acc;
     }
   }
   __syncthreads();

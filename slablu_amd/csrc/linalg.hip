@@ -10,32 +10,6 @@
 
 namespace slab {
 
-// ---------------------------------------------------------------------------
-// MFMA wrappers: 16x16x4 tiles.  Lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; accumulator element q of lane l is D[row(l, q)][l%16] with
-//   FP32 (v_mfma_f32_16x16x4f32): row = 4*(l/16) + q
-//   FP64 (v_mfma_f64_16x16x4f64): row = 4*q + l/16
-// (both measured on gfx950 with single-lane probes).
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef double acc_t __attribute__((ext_vector_type(4)));
-  __device__ static inline acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lane, int q) { return 4 * q + (lane >> 4); }
-};
-template <>
-struct Mfma<float> {
-  typedef float acc_t __attribute__((ext_vector_type(4)));
-  __device__ static inline acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lane, int q) { return 4 * (lane >> 4) + q; }
-};
-
 static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 static inline int grid_batch(int batch) { return batch < 65535 ? batch : 65535; }
 
@@ -1011,11 +985,41 @@ size_t getri_ws_bytes(int batch, int n, size_t elem) {
 }
 
 template <typename T>
-hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> Xo, void* ws, hipStream_t s) {
-  if (batch <= 0 || n <= 0) return hipSuccess;
-  if (batch > 32767 || n > 16384) return hipErrorInvalidValue;
+__global__ __launch_bounds__(256) void k_getri_split(int n, int np, int nb, const T* X, const int* iperm, View<T> Li,
+                                                     View<T> Ui) {
+  __shared__ T t[32][33];
+  const int b = blockIdx.z, r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t nn = (int64_t)np * np;
+  const T* XL = X + b * nn;
+  const T* XU = X + (nb + b) * nn;
+  const int* q = iperm + (int64_t)b * n;
+  // Li = L^{-1} P: column j of it is column iperm[j] of L^{-1}
+#pragma unroll
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    if (r < n && c < n) *Li.at(b, r, c) = XL[(int64_t)r * np + q[c]];
+  }
+  // Ui = U^{-1} = (X[nb + b])^T through an LDS tile
+#pragma unroll
+  for (int i = ty; i < 32; i += 8) t[i][tx] = XU[(int64_t)(c0 + i) * np + r0 + tx];
+  __syncthreads();
+#pragma unroll
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    if (r < n && c < n) *Ui.at(b, r, c) = t[tx][i];
+  }
+}
+
+// The triangular inverses of getri in the workspace: X[z] = L^{-1} of matrix
+// z (z < batch), X[batch + z] = (U^T)^{-1}; iperm of every matrix.  Returns
+// the padded size np.
+template <typename T>
+static hipError_t getri_core(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, void* ws, hipStream_t s,
+                             int* np_out) {
   int np = 32;
   while (np < n) np *= 2;
+  *np_out = np;
   const int64_t nn = (int64_t)np * np;
   T* W = reinterpret_cast<T*>(ws);
   T* X = W + 2 * batch * nn;
@@ -1044,11 +1048,41 @@ hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, Vie
     e = gemm_outer<T>(z, nn, nn / 4, nn, npairs, sb, sb, sb, T(-1), X22, Tv, T(0), X21, s);
     if (e != hipSuccess) return e;
   }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> Xo, void* ws, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (batch > 32767 || n > 16384) return hipErrorInvalidValue;
+  int np;
+  hipError_t e = getri_core<T>(batch, n, LU, ipiv, pbs, ws, s, &np);
+  if (e != hipSuccess) return e;
+  const int64_t nn = (int64_t)np * np;
+  T* W = reinterpret_cast<T*>(ws);
+  T* X = W + 2 * batch * nn;
+  const int* iperm = reinterpret_cast<const int*>(X + 2 * batch * nn + 2 * batch * (nn / 4));
   // Y = U^{-1} L^{-1} into W[0..batch) (the copies of LU are no longer read)
   e = gemm<T>(batch, n, n, n, T(1), View<T>{X + batch * nn, 1, np, nn}, View<T>{X, np, 1, nn}, T(0),
               View<T>{W, np, 1, nn}, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_getri_gather<T>, dim3(n, batch), dim3(256), 0, s, n, np, (const T*)W, (const int*)iperm, Xo);
+  hipLaunchKernelGGL(k_getri_gather<T>, dim3(n, batch), dim3(256), 0, s, n, np, (const T*)W, iperm, Xo);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t getri_factors(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> Li, View<T> Ui,
+                         void* ws, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (batch > 32767 || n > 16384) return hipErrorInvalidValue;
+  int np;
+  hipError_t e = getri_core<T>(batch, n, LU, ipiv, pbs, ws, s, &np);
+  if (e != hipSuccess) return e;
+  const int64_t nn = (int64_t)np * np;
+  const T* X = reinterpret_cast<const T*>(ws) + 2 * batch * nn;
+  const int* iperm = reinterpret_cast<const int*>(X + 2 * batch * nn + 2 * batch * (nn / 4));
+  const int nt = (n + 31) / 32;
+  hipLaunchKernelGGL(k_getri_split<T>, dim3(nt, nt, batch), dim3(256), 0, s, n, np, batch, X, iperm, Li, Ui);
   return hipGetLastError();
 }
 
@@ -1060,7 +1094,9 @@ hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, Vie
   template hipError_t getrs<T>(int, int, int, bool, View<T>, const int*, int64_t, View<T>, hipStream_t);         \
   template hipError_t copy_batched<T>(int, int, int, View<T>, View<T>, hipStream_t);                              \
   template hipError_t set_identity<T>(int, int, View<T>, hipStream_t);                                            \
-  template hipError_t getri<T>(int, int, View<T>, const int*, int64_t, View<T>, void*, hipStream_t);
+  template hipError_t getri<T>(int, int, View<T>, const int*, int64_t, View<T>, void*, hipStream_t);             \
+  template hipError_t getri_factors<T>(int, int, View<T>, const int*, int64_t, View<T>, View<T>, void*,          \
+                                       hipStream_t);
 SLAB_INSTANTIATE(double)
 SLAB_INSTANTIATE(float)
 

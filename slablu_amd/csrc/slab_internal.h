@@ -80,6 +80,11 @@ hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
 size_t getri_ws_bytes(int batch, int n, size_t elem);
 template <typename T>
 hipError_t getri(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> X, void* ws, hipStream_t s);
+// The two triangular inverses instead of their product: Li = L^{-1} P and
+// Ui = U^{-1} (Ui may be LU itself; same workspace as getri)
+template <typename T>
+hipError_t getri_factors(int batch, int n, View<T> LU, const int* ipiv, int64_t pbs, View<T> Li, View<T> Ui,
+                         void* ws, hipStream_t s);
 // getrs for nrhs <= 8, non-transposed, one workgroup per matrix
 // LU solve with the composed pivot permutation (perm from piv_to_perm_batched),
 // nrhs <= 8, one wavefront per matrix
@@ -108,10 +113,13 @@ template <typename T>
 hipError_t btri_lu_factor_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                   int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs,
                                   int64_t pis, int* info, hipStream_t s);
+// linv (optional): L^{-1} P of every diagonal block in diag's layout, diag
+// then holding U^{-1} (getri_factors): every sweep stage is a block product
 template <typename T>
 hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                  int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
-                                 int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s);
+                                 int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s,
+                                 const T* linv = nullptr);
 // block-Thomas factorization / sweeps with explicit diagonal inverses
 template <typename T>
 bool btri_inv_fused_ok(int bs);

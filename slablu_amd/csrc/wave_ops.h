@@ -137,4 +137,28 @@ __device__ __attribute__((always_inline)) inline void static_for(F&& f) {
   }
 }
 
+// MFMA wrappers: 16x16x4 tiles.  Lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; accumulator element q of lane l is D[row(l, q)][l%16] with
+//   FP32 (v_mfma_f32_16x16x4f32): row = 4*(l/16) + q
+//   FP64 (v_mfma_f64_16x16x4f64): row = 4*q + l/16
+// (both measured on gfx950 with single-lane probes).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int q) { return 4 * q + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int q) { return 4 * (lane >> 4) + q; }
+};
+
 }  // namespace slab

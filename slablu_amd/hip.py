@@ -44,6 +44,9 @@ def _load():
         "slab_btri_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                              _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
         "slab_btri_inv_fused": ([_int, _int], _int),
+        "slab_btri_invert_diag": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp], _int),
+        "slab_btri_solve_twisted_inv": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64,
+                                         _i64, _vp, _i64, _i64, _int, _vp], _int),
         "slab_btri_fused": ([_int, _int], _int),
         "slab_btri_factor_twisted": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                                       _i64, _i64, _vp, _vp], _int),
@@ -239,10 +242,33 @@ def btri_factor(diag, sub, sup, ipiv, info=None, twisted=False):
               _stream()), "slab_btri_factor_twisted" if twisted else "slab_btri_factor")
 
 
-def btri_solve(diag, sub, sup, ipiv, f, transpose=False, twisted=False):
-    """In-place sweeps on f (b, m*bs, nrhs) (row-major rows)."""
+def btri_invert_diag(diag, ipiv, linv):
+    """After btri_factor: diag blocks <- U^{-1}, linv (same shape) <- L^{-1} P;
+    btri_solve(..., linv=linv) then sweeps with block products."""
+    b, m, bs, _ = diag.shape
+    _require(diag.is_contiguous() and linv.is_contiguous() and linv.shape == diag.shape and
+             linv.dtype == diag.dtype and ipiv.is_contiguous() and ipiv.shape == (b, m, bs),
+             "btri_invert_diag layouts")
+    ws = torch.empty(int(LIB.slab_getri_ws(_dt(diag), b * m, bs)), dtype=torch.uint8, device=diag.device)
+    _check(LIB.slab_btri_invert_diag(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(ipiv),
+                                     ipiv.stride(0), ipiv.stride(1), _p(linv), _p(ws), _stream()),
+           "slab_btri_invert_diag")
+    return linv
+
+
+def btri_solve(diag, sub, sup, ipiv, f, transpose=False, twisted=False, linv=None):
+    """In-place sweeps on f (b, m*bs, nrhs) (row-major rows); linv: the
+    inverse factors of btri_invert_diag (twisted factors only)."""
     b, m, bs, _ = diag.shape
     _require(f.dim() == 3 and f.shape[0] == b and f.shape[1] == m * bs and f.stride(2) == 1, "btri rhs layout")
+    if linv is not None:
+        _require(twisted and not transpose, "inverse-factor sweeps need twisted factors")
+        _check(LIB.slab_btri_solve_twisted_inv(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1),
+                                               _p(linv), _p(sub), sub.stride(0), sub.stride(1) if m > 1 else 0,
+                                               _p(sup), sup.stride(0), sup.stride(1) if m > 1 else 0, _p(f),
+                                               f.stride(0), f.stride(1), f.shape[2], _stream()),
+               "slab_btri_solve_twisted_inv")
+        return f
     args = (_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub), sub.stride(0),
             sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0), sup.stride(1) if m > 1 else 0, _p(ipiv),
             ipiv.stride(0), ipiv.stride(1), _p(f), f.stride(0), f.stride(1), f.shape[2])

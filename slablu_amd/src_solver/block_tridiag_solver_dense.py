@@ -44,17 +44,23 @@ def unpack_pivs(b, m, bs, Tdata, device=None):
 
 
 class Block_Tridiagonal:
-    def __init__(self, b, m, bs, form="lu", twisted=False):
+    def __init__(self, b, m, bs, form="lu", twisted=False, inv_factors=False):
         """form 'lu': the reference's block LU (packed LU + Lfac/Ufac + pivots
         in Tdata); 'inv': block-Thomas with explicit diagonal inverses.
         twisted (form 'lu'): eliminate from both ends of each chain towards
         block m//2, which halves the dependent steps of the factorization and
-        of every sweep; Tdata then holds the twisted factors (same layout)."""
+        of every sweep; Tdata then holds the twisted factors (same layout).
+        inv_factors (twisted): the same elimination, but each diagonal block
+        is also kept as its triangular inverses L^{-1} P and U^{-1}, so the
+        non-transposed sweeps are block products (matrix cores) instead of
+        triangular substitutions."""
         if form not in ("inv", "lu"):
             raise ValueError("form must be 'inv' or 'lu'")
         self.b, self.m, self.bs = b, m, bs
         self.form = form
         self.twisted = bool(twisted) and form == "lu"
+        self.inv_factors = bool(inv_factors) and self.twisted
+        self.linv = self.uinv = None
         self.pivdata_nblocks = int(m / bs) + 1
         self.storage_blocks = m + 2 * (m - 1) + self.pivdata_nblocks
         self.nbytes = (self.b * self.storage_blocks * self.bs ** 2) * 8
@@ -95,6 +101,11 @@ class Block_Tridiagonal:
             self.twisted = False
         hip.btri_factor(d, l, u, self.pivs, info=self.info, twisted=self.twisted)
         pack_pivs(b, m, bs, self.Tdata, self.pivs)
+        if self.inv_factors and self.twisted:
+            self.uinv = d.contiguous().clone()
+            self.linv = torch.empty_like(self.uinv)
+            hip.btri_invert_diag(self.uinv, self.pivs, self.linv)
+            self.nbytes += 2 * self.uinv.numel() * self.itemsize
 
     def lusolve_sweep(self, f, b_start=None, b_end=None, transpose=False):
         """Solve (in place on f, (b_end-b_start, m*bs, nrhs)); returns f."""
@@ -113,7 +124,11 @@ class Block_Tridiagonal:
         else:
             if transpose and self.twisted:
                 raise NotImplementedError("transposed sweeps need the non-twisted factorization")
-            hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose, twisted=self.twisted)
+            if self.linv is not None:
+                hip.btri_solve(self.uinv[b_start:b_end], l, u, None, fv, twisted=True,
+                               linv=self.linv[b_start:b_end])
+            else:
+                hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose, twisted=self.twisted)
         return fv
 
 

@@ -307,7 +307,10 @@ class Domain_Solver:
         diag = Arr_d - CW[:, :mr, 1, :, 1, :] - CW[:, 1:, 0, :, 0, :]
         sup = Arr_u - CW[:, 1:mr, 0, :, 1, :]
         sub = Arr_l - CW[:, 1:mr, 1, :, 0, :]
-        self.Ared_solver = Block_Tridiagonal(Np, mr, B, twisted=True)
+        # FP32 (mixed precision): triangular inverses of the diagonal blocks,
+        # so the T_slab and refinement sweeps are block products; the FP64
+        # path keeps the substitutions
+        self.Ared_solver = Block_Tridiagonal(Np, mr, B, twisted=True, inv_factors=self.precision == "mixed")
         self.Ared_solver.build_sweep(diag.contiguous(), sub.contiguous(), sup.contiguous())
 
     def _slab_u_local(self, upad, s):

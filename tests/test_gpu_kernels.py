@@ -239,6 +239,40 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (2, 2, 37), (2, 11, 100), (3, 4, 100)])
+@pytest.mark.parametrize("nrhs", [1, 3, 32, 70])
+def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs):
+    """Twisted block LU whose diagonal blocks are applied through their
+    triangular inverses (L^{-1} P, U^{-1}): the same solution as the
+    substitution sweeps and as a dense solve."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(b * 1000 + m * 100 + bs + nrhs)
+    diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
+    sub0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    sup0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
+    f0 = torch.randn(b, m * bs, nrhs, generator=g, dtype=torch.float64)
+    diag, sub, sup = (t.to("cuda", dtype).contiguous() for t in (diag0, sub0, sup0))
+    ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
+    hip.btri_factor(diag, sub, sup, ipiv, twisted=True)
+    f_lu = f0.to("cuda", dtype).contiguous()
+    hip.btri_solve(diag, sub, sup, ipiv, f_lu, twisted=True)
+    uinv = diag.clone()
+    linv = torch.empty_like(uinv)
+    hip.btri_invert_diag(uinv, ipiv, linv)
+    f = f0.to("cuda", dtype).contiguous()
+    hip.btri_solve(uinv, sub, sup, None, f, twisted=True, linv=linv)
+    M = _btri_dense(diag0, sub0[:, :m - 1], sup0[:, :m - 1])
+    ref = torch.linalg.solve(M, f0)
+    torch.cuda.synchronize()
+    assert _rel(f, ref) < 200 * TOL[dtype]
+    assert _rel(f, f_lu.double().cpu()) < 200 * TOL[dtype]
+    # U^{-1} is upper triangular and L^{-1} P has the unit lower triangle
+    # of L^{-1} in its permuted columns: L^{-1} P P^T L = I on every block
+    Ui = uinv.double().cpu()
+    assert torch.count_nonzero(torch.tril(Ui, -1)) == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("twisted", [False, True])
 @pytest.mark.parametrize("nrhs", [1, 70])
 def test_btri_solve_block_sparse_rhs(dtype, twisted, nrhs):
