@@ -243,7 +243,10 @@ class Domain_Solver:
     def build_Aloc_solver(self):
         Np, n1, f = self.Npan, self.Npan_loc, self.hps.p - 2
         diag, sub, sup = self.elim_blocks()
-        self.Aloc_solver = Block_Tridiagonal(Np * n1, self.buf_pans - 1, f)
+        # FP32 (mixed precision): twisted elimination, half the dependent
+        # block steps in the factorization and in every refinement sweep
+        mixed = self.precision == "mixed"
+        self.Aloc_solver = Block_Tridiagonal(Np * n1, self.buf_pans - 1, f, twisted=mixed)
         self.Aloc_solver.build_sweep(diag, sub, sup)
 
     def _couplings(self):
