@@ -75,6 +75,16 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    // C is read before the K loop so that its latency overlaps the loop
+    T cv[2][2][4];
+    if (beta != T(0)) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cv[i][j][q] = Cb[c_row[i][q] + c_col[j]];
+    }
     // operands of the next K chunk travel in registers while the current
     // chunk is multiplied
     T ra[NU], rb[NU];
@@ -122,7 +132,7 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
           const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
           if (row < m && col < n) {
             T v = alpha * acc[i][j][q];
-            if (beta != T(0)) v += beta * Cb[c_row[i][q] + c_col[j]];
+            if (beta != T(0)) v += beta * cv[i][j][q];
             Cb[c_row[i][q] + c_col[j]] = v;
           }
         }
