@@ -20,10 +20,15 @@ static inline int grid_batch(int batch) { return batch < 65535 ? batch : 65535; 
 // ---------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
-template <typename T, typename I>
+// VEC: A and B row-major with 16-byte aligned rows and k, n multiples of
+// the vector width: each thread moves 16-byte vectors of A (along k) and B
+// (along n) instead of single elements.
+template <typename T, typename I, bool VEC>
 __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
                                               T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co) {
   typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int VW = VEC ? 16 / (int)sizeof(T) : 1;
+  typedef T vec_t __attribute__((ext_vector_type(VW)));
   __shared__ T As[GBK][GBM + 1];
   __shared__ T Bs[GBK][GBN + 1];
   const int tile = blockIdx.x;
@@ -34,21 +39,29 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
   const bool b_rowmajor = (B.cs == 1);
   const int mm1 = m - 1, nm1 = n - 1, km1 = k - 1;
   constexpr int NU = GBM * GBK / 256;
+  constexpr int NV = NU / VW;  // loads per operand per thread
   // element offsets inside one matrix in I (32-bit whenever the host has
   // checked that they fit); row / column parts fixed per thread, so only the
   // K index moves between chunks.  Rows of A beyond m and columns of B
   // beyond n are read clamped and feed only C entries that are never
   // stored; K beyond k is zeroed.
-  I a_off[NU], b_off[NU];
-  int a_k[NU], b_k[NU];
+  I a_off[NV], b_off[NV];
+  int a_k[NV], b_k[NV];
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
+  for (int u = 0; u < NV; ++u) {
     const int e = tid + 256 * u;
     int r, kk, c, kb;
-    if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
-    if (b_rowmajor) { kb = e / GBN; c = e % GBN; } else { kb = e % GBK; c = e / GBK; }
-    a_off[u] = (I)(m0 + r < mm1 ? m0 + r : mm1) * (I)A.rs;
-    b_off[u] = (I)(n0 + c < nm1 ? n0 + c : nm1) * (I)B.cs;
+    if (VEC) {
+      r = e / (GBK / VW); kk = (e % (GBK / VW)) * VW;
+      kb = e / (GBN / VW); c = (e % (GBN / VW)) * VW;
+      a_off[u] = (I)(m0 + r < mm1 ? m0 + r : mm1) * (I)A.rs;
+      b_off[u] = (I)(n0 + c < n ? n0 + c : n - VW);
+    } else {
+      if (a_rowmajor) { r = e / GBK; kk = e % GBK; } else { r = e % GBM; kk = e / GBM; }
+      if (b_rowmajor) { kb = e / GBN; c = e % GBN; } else { kb = e % GBK; c = e / GBK; }
+      a_off[u] = (I)(m0 + r < mm1 ? m0 + r : mm1) * (I)A.rs;
+      b_off[u] = (I)(n0 + c < nm1 ? n0 + c : nm1) * (I)B.cs;
+    }
     a_k[u] = kk;
     b_k[u] = kb;
   }
@@ -90,21 +103,46 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
     T ra[NU], rb[NU];
     auto load = [&](int k0) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
+      for (int u = 0; u < NV; ++u) {
         const int gk = k0 + a_k[u], gkb = k0 + b_k[u];
-        const T va = Ab[a_off[u] + (I)(gk < km1 ? gk : km1) * (I)A.cs];
-        ra[u] = (gk < k) ? va : T(0);
-        const T vb = Bb[b_off[u] + (I)(gkb < km1 ? gkb : km1) * (I)B.rs];
-        rb[u] = (gkb < k) ? vb : T(0);
+        if constexpr (VEC) {
+          // k is a multiple of VW: a vector is wholly inside or past k
+          const vec_t va = *reinterpret_cast<const vec_t*>(Ab + a_off[u] + (I)(gk < k ? gk : k - VW));
+          const vec_t vb = *reinterpret_cast<const vec_t*>(Bb + b_off[u] + (I)(gkb < km1 ? gkb : km1) * (I)B.rs);
+#pragma unroll
+          for (int w = 0; w < VW; ++w) {
+            ra[u * VW + w] = (gk < k) ? va[w] : T(0);
+            rb[u * VW + w] = (gkb < k) ? vb[w] : T(0);
+          }
+        } else {
+          const T va = Ab[a_off[u] + (I)(gk < km1 ? gk : km1) * (I)A.cs];
+          ra[u] = (gk < k) ? va : T(0);
+          const T vb = Bb[b_off[u] + (I)(gkb < km1 ? gkb : km1) * (I)B.rs];
+          rb[u] = (gkb < k) ? vb : T(0);
+        }
       }
     };
     if (k > 0) load(0);
     for (int k0 = 0; k0 < k; k0 += GBK) {
+      if constexpr (VEC) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int e = tid + 256 * u;
-        if (a_rowmajor) As[e % GBK][e / GBK] = ra[u]; else As[e / GBM][e % GBM] = ra[u];
-        if (b_rowmajor) Bs[e / GBN][e % GBN] = rb[u]; else Bs[e % GBK][e / GBK] = rb[u];
+        for (int u = 0; u < NV; ++u) {
+          const int e = tid + 256 * u;
+          const int r = e / (GBK / VW), kk = (e % (GBK / VW)) * VW;
+          const int kb = e / (GBN / VW), c = (e % (GBN / VW)) * VW;
+#pragma unroll
+          for (int w = 0; w < VW; ++w) {
+            As[kk + w][r] = ra[u * VW + w];
+            Bs[kb][c + w] = rb[u * VW + w];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const int e = tid + 256 * u;
+          if (a_rowmajor) As[e % GBK][e / GBK] = ra[u]; else As[e / GBM][e % GBM] = ra[u];
+          if (b_rowmajor) Bs[e / GBN][e % GBN] = rb[u]; else Bs[e % GBK][e / GBK] = rb[u];
+        }
       }
       __syncthreads();
       if (k0 + GBK < k) load(k0 + GBK);
@@ -195,12 +233,23 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
     return r * (V.rs < 0 ? -V.rs : V.rs) + c * (V.cs < 0 ? -V.cs : V.cs);
   };
   const int64_t lim = INT_MAX;
-  if (span(m, k, A) < lim && span(k, n, B) < lim && span(m, n, C) < lim)
-    hipLaunchKernelGGL((k_gemm<T, int>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao, bo,
-                       co);
-  else
-    hipLaunchKernelGGL((k_gemm<T, int64_t>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
+  // 16-byte vectors of A and B: row-major, aligned rows and batch offsets,
+  // k and n multiples of the vector width
+  constexpr int VW = 16 / (int)sizeof(T);
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const bool vec = A.cs == 1 && B.cs == 1 && A.rs % VW == 0 && B.rs % VW == 0 && A.bs % VW == 0 &&
+                   B.bs % VW == 0 && ao % VW == 0 && bo % VW == 0 && al(A.p) && al(B.p) && k % VW == 0 &&
+                   n % VW == 0 && k > 0 && (sizeof(T) == 4 || k >= 64);  // FP64 rank-16: measured slower
+  const bool small = span(m, k, A) < lim && span(k, n, B) < lim && span(m, n, C) < lim;
+  if (small && vec)
+    hipLaunchKernelGGL((k_gemm<T, int, true>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
                        bo, co);
+  else if (small)
+    hipLaunchKernelGGL((k_gemm<T, int, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
+                       bo, co);
+  else
+    hipLaunchKernelGGL((k_gemm<T, int64_t, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn,
+                       ao, bo, co);
   return hipGetLastError();
 }
 

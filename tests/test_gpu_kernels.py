@@ -42,6 +42,42 @@ def test_gemm(dtype, m, n, k, ta, tb):
     assert _rel(C, ref) < TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("k0", [0, 32, 96, 352])
+def test_gemm_leaf_update_views(dtype, k0):
+    """The leaf LU's trailing update on strided sub-views of a batch of
+    400 x 480 matrices (16-byte vector loads whenever rows, offsets, k and n
+    allow them; partial 64-row tiles at the bottom edge)."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(k0 + 7)
+    M0 = torch.randn(5, 400, 480, generator=g, dtype=torch.float64)
+    M = M0.to("cuda", dtype)
+    A, B, C = M[:, k0 + 32:, k0:k0 + 32], M[:, k0:k0 + 32, k0 + 32:], M[:, k0 + 32:, k0 + 32:]
+    ref = M0[:, k0 + 32:, k0 + 32:] - 0.5 * M0[:, k0 + 32:, k0:k0 + 32] @ M0[:, k0:k0 + 32, k0 + 32:]
+    hip.gemm(A, B, C, alpha=-0.5, beta=1.0)
+    torch.cuda.synchronize()
+    assert _rel(C, ref) < TOL[dtype]
+    # the rest of the matrix is untouched
+    assert torch.equal(M[:, :k0 + 32].cpu(), M0[:, :k0 + 32].to(dtype))
+    assert torch.equal(M[:, k0 + 32:, :k0 + 32].cpu(), M0[:, k0 + 32:, :k0 + 32].to(dtype))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,n,k", [(100, 36, 32), (64, 4, 8), (7, 128, 4), (130, 68, 44)])
+def test_gemm_vector_shapes(dtype, m, n, k):
+    """Shapes whose k and n are multiples of the vector width, with partial
+    tiles in every direction, against the FP64 product."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(m + n + k)
+    A0 = torch.randn(3, m, k, generator=g, dtype=torch.float64)
+    B0 = torch.randn(3, k, n, generator=g, dtype=torch.float64)
+    C0 = torch.randn(3, m, n, generator=g, dtype=torch.float64)
+    C = C0.to("cuda", dtype)
+    hip.gemm(A0.to("cuda", dtype), B0.to("cuda", dtype), C, alpha=1.5, beta=-1.0)
+    torch.cuda.synchronize()
+    assert _rel(C, 1.5 * A0 @ B0 - C0) < TOL[dtype]
+
+
 def test_gemm_broadcast_and_zero_beta():
     hip = _hip()
     g = torch.Generator().manual_seed(1)
