@@ -22,8 +22,10 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 
 // VEC: A and B row-major with 16-byte aligned rows and k, n multiples of
 // the vector width: each thread moves 16-byte vectors of A (along k) and B
-// (along n) instead of single elements.
-template <typename T, typename I, bool VEC>
+// (along n) instead of single elements.  VC (with VEC): C row-major and
+// aligned too; the C tile is read as row vectors before the K loop and the
+// result leaves through an LDS tile, so C moves as 16-byte row vectors.
+template <typename T, typename I, bool VEC, bool VC>
 __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
                                               T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co) {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -40,6 +42,8 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
   const int mm1 = m - 1, nm1 = n - 1, km1 = k - 1;
   constexpr int NU = GBM * GBK / 256;
   constexpr int NV = NU / VW;  // loads per operand per thread
+  constexpr int CPR = GBN / VW, NCV = GBM * GBN / VW / 256;  // C vectors per row / per thread
+  __shared__ __attribute__((aligned(16))) T Cs[VC ? GBM : 1][VC ? GBN + VW : 1];
   // element offsets inside one matrix in I (32-bit whenever the host has
   // checked that they fit); row / column parts fixed per thread, so only the
   // K index moves between chunks.  Rows of A beyond m and columns of B
@@ -64,6 +68,14 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
     }
     a_k[u] = kk;
     b_k[u] = kb;
+  }
+  I cvo[VC ? NCV : 1];
+  if constexpr (VC) {
+#pragma unroll
+    for (int u = 0; u < NCV; ++u) {
+      const int e = tid + 256 * u, cr = e / CPR, cc = (e % CPR) * VW;
+      cvo[u] = (I)(m0 + cr < mm1 ? m0 + cr : mm1) * (I)C.rs + (I)(n0 + cc < n ? n0 + cc : n - VW);
+    }
   }
   I c_row[2][4], c_col[2];
 #pragma unroll
@@ -90,7 +102,11 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     // C is read before the K loop so that its latency overlaps the loop
     T cv[2][2][4];
-    if (beta != T(0)) {
+    vec_t cvv[VC ? NCV : 1];
+    if (VC && beta != T(0)) {
+#pragma unroll
+      for (int u = 0; u < (VC ? NCV : 1); ++u) cvv[u] = *reinterpret_cast<const vec_t*>(Cb + cvo[u]);
+    } else if (!VC && beta != T(0)) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -160,21 +176,42 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
       }
       __syncthreads();
     }
+    if constexpr (VC) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
-          if (row < m && col < n) {
-            T v = alpha * acc[i][j][q];
-            if (beta != T(0)) v += beta * cv[i][j][q];
-            Cb[c_row[i][q] + c_col[j]] = v;
+          for (int q = 0; q < 4; ++q)
+            Cs[wm * 32 + i * 16 + Mfma<T>::row(lane, q)][wn * 32 + j * 16 + (lane & 15)] = acc[i][j][q];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < NCV; ++u) {
+        const int e = tid + 256 * u, cr = e / CPR, cc = (e % CPR) * VW;
+        vec_t v = *reinterpret_cast<const vec_t*>(&Cs[cr][cc]);
+#pragma unroll
+        for (int w = 0; w < VW; ++w) v[w] = (beta != T(0)) ? alpha * v[w] + beta * cvv[u][w] : alpha * v[w];
+        // n is a multiple of VW: a vector is wholly inside or past n
+        if (m0 + cr < m && n0 + cc < n) *reinterpret_cast<vec_t*>(Cb + cvo[u]) = v;
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, q);
+            if (row < m && col < n) {
+              T v = alpha * acc[i][j][q];
+              if (beta != T(0)) v += beta * cv[i][j][q];
+              Cb[c_row[i][q] + c_col[j]] = v;
+            }
           }
         }
-      }
+    }
   }
 }
 
@@ -240,16 +277,22 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   const bool vec = A.cs == 1 && B.cs == 1 && A.rs % VW == 0 && B.rs % VW == 0 && A.bs % VW == 0 &&
                    B.bs % VW == 0 && ao % VW == 0 && bo % VW == 0 && al(A.p) && al(B.p) && k % VW == 0 &&
                    n % VW == 0 && k > 0 && (sizeof(T) == 4 || k >= 64);  // FP64 rank-16: measured slower
+  // (FP64 products measured slower through the 33 KB C tile)
+  const bool vc = vec && sizeof(T) == 4 && C.cs == 1 && C.rs % VW == 0 && C.bs % VW == 0 && co % VW == 0 &&
+                  al(C.p);
   const bool small = span(m, k, A) < lim && span(k, n, B) < lim && span(m, n, C) < lim;
-  if (small && vec)
-    hipLaunchKernelGGL((k_gemm<T, int, true>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
-                       bo, co);
-  else if (small)
-    hipLaunchKernelGGL((k_gemm<T, int, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn, ao,
-                       bo, co);
-  else
-    hipLaunchKernelGGL((k_gemm<T, int64_t, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn,
+  if (small && vc)
+    hipLaunchKernelGGL((k_gemm<T, int, true, true>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn,
                        ao, bo, co);
+  else if (small && vec)
+    hipLaunchKernelGGL((k_gemm<T, int, true, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C,
+                       tn, ao, bo, co);
+  else if (small)
+    hipLaunchKernelGGL((k_gemm<T, int, false, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C,
+                       tn, ao, bo, co);
+  else
+    hipLaunchKernelGGL((k_gemm<T, int64_t, false, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta,
+                       C, tn, ao, bo, co);
   return hipGetLastError();
 }
 
