@@ -887,7 +887,7 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
   // 256-thread workgroups (several per CU), up to 1024 rows in 512-thread
   // workgroups (two wavefronts per SIMD), four rows per thread up to 2048
   if (nrows <= 256) return getrf_reg<T, 32, 1, 256>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 512) return getrf_reg<T, 32, 2, 256>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  if (nrows <= 512) return getrf_reg<T, 32, 1, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   if (nrows <= 1024) return getrf_reg<T, 16, 2, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   if (nrows <= 2048) return getrf_reg<T, 8, 4, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
   // taller panels: LDS-resident panel, generic interchanges
