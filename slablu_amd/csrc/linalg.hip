@@ -254,6 +254,49 @@ __global__ __launch_bounds__(256) void k_gemv(int batch, int m, int n, int k, T 
   }
 }
 
+// Short rows (k <= 256): 16 lanes per output row, four rows per wavefront,
+// so a row of 80 columns keeps every lane busy (the leaf recovery
+// u_c -= S' d_x reads 400 x 80 per leaf).
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_gemv16(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
+                                                T beta, View<T> C) {
+  constexpr int LPR = 16;
+  const int l = threadIdx.x & (LPR - 1), grp = threadIdx.x / LPR;
+  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+    for (int row0 = blockIdx.x * (256 / LPR); row0 < m; row0 += gridDim.x * (256 / LPR)) {
+      const int row = row0 + grp;
+      const int rc = row < m ? row : m - 1;
+      T acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = T(0);
+      const T* arow = A.at(b, rc, 0);
+      const int nm1 = n - 1;
+#pragma unroll 4
+      for (int kk = l; kk < k; kk += LPR) {
+        const T a = arow[kk];
+        const T* brow = B.at(b, kk, 0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += a * brow[(int64_t)(c < nm1 ? c : nm1) * B.cs];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off);
+      }
+      if (row < m && l < n) {
+        T v = T(0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c == l) v = acc[c];
+        T* cp = C.at(b, row, l);
+        T out = alpha * v;
+        if (beta != T(0)) out += beta * (*cp);
+        *cp = out;
+      }
+    }
+  }
+}
+
 // Two batch levels: `outer` groups of `batch` products, group g offset by
 // g * (ao, bo, co) elements from the views.
 template <typename T>
@@ -299,6 +342,17 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
 template <typename T>
 hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  if (n <= 8 && A.cs == 1 && k >= 32 && k <= 256) {
+    const int gx = (m + 15) / 16 < 8192 ? (m + 15) / 16 : 8192;
+    dim3 grid(gx, grid_batch(batch));
+    if (n <= 1)
+      hipLaunchKernelGGL((k_gemv16<T, 1>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    else if (n <= 4)
+      hipLaunchKernelGGL((k_gemv16<T, 4>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    else
+      hipLaunchKernelGGL((k_gemv16<T, 8>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C);
+    return hipGetLastError();
+  }
   if (n <= 8 && A.cs == 1 && k >= 32) {
     const int gx = (m + 3) / 4 < 8192 ? (m + 3) / 4 : 8192;
     dim3 grid(gx, grid_batch(batch));

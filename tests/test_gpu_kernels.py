@@ -78,6 +78,25 @@ def test_gemm_vector_shapes(dtype, m, n, k):
     assert _rel(C, 1.5 * A0 @ B0 - C0) < TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,k", [(5, 32), (400, 80), (1001, 200), (37, 256), (64, 257), (130, 1000)])
+@pytest.mark.parametrize("n", [1, 3, 8])
+def test_gemv_paths(dtype, m, k, n):
+    """Products with at most 8 right-hand columns (the 16-lanes-per-row
+    kernel for k <= 256, 64 lanes per row above), with A a column slice of
+    a wider matrix as in the leaf recovery, against the FP64 product."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(m * 7 + k + n)
+    W0 = torch.randn(3, m, k + 40, generator=g, dtype=torch.float64)
+    B0 = torch.randn(3, k, n, generator=g, dtype=torch.float64)
+    C0 = torch.randn(3, m, n, generator=g, dtype=torch.float64)
+    W = W0.to("cuda", dtype)
+    C = C0.to("cuda", dtype)
+    hip.gemm(W[:, :, 40:], B0.to("cuda", dtype), C, alpha=-1.0, beta=0.5)
+    torch.cuda.synchronize()
+    assert _rel(C, -W0[:, :, 40:] @ B0 + 0.5 * C0) < TOL[dtype]
+
+
 def test_gemm_broadcast_and_zero_beta():
     hip = _hip()
     g = torch.Generator().manual_seed(1)
