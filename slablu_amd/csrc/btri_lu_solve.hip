@@ -258,9 +258,146 @@ hipError_t launch_lu_solve_pf(int bs, const ChainSet<T>& c0, const ChainSet<T>& 
   return launch_lu_solve<T, NR, PFBIG>(bs, c0, c1, ldf, nrhs, mode, s);
 }
 
+// Single-column sweeps with inverse factors (FP32, bs <= 104, bs % 4 == 0):
+// every stage is out = M in (DIAGL: L^{-1} P, DIAGU: U^{-1}) or
+// out = f - M in (SUB: Lfac, SUP: Ufac), so no block passes through LDS.
+// Thread (r, h) = (tid / 2, tid % 2) holds half of row r of the stage's
+// block in registers as 16-byte vectors, loaded one stage ahead; the
+// vectors in and out are 3 rotating LDS buffers, one barrier per stage.
+// Same stage order, modes and chain sets as k_btri_lu_solve.
+constexpr int kInv1MaxBs = 104;
+
+__global__ __launch_bounds__(256) void k_btri_inv1(int bs, ChainSet<float> c0, ChainSet<float> c1, int64_t ldf,
+                                                   int mode) {
+  constexpr int NVMAX = kInv1MaxBs / 2 / 4;  // float4 per half row
+  __shared__ __attribute__((aligned(16))) float vb[3][kInv1MaxBs];
+  const int tid = threadIdx.x, r = tid >> 1, h = tid & 1;
+  const int hw = ((bs + 7) / 8) * 4;  // first half: a multiple of 4 columns
+  const int cb = h ? hw : 0, ce = h ? bs : hw;
+  const int nv = (ce - cb) >> 2;      // bs % 4 == 0
+  const int rc = r < bs ? r : bs - 1;
+  const int nt = c0.nchains + c1.nchains;
+  for (int g = blockIdx.x; g < nt; g += gridDim.x) {
+    const bool sec = g >= c0.nchains;
+    const ChainSet<float> a = sec ? c1 : c0;
+    const int64_t b = sec ? g - c0.nchains : g;
+    const int m = a.m;
+    const int nst = mode == 0 ? 4 * m - 2 : (mode == 1 ? 2 * m - 1 : 2 * m);
+    auto decode = [&](int s, int& i) __attribute__((always_inline)) -> int {
+      if (mode == 2) {
+        if (s & 1) { i = m - 1 - ((s - 1) >> 1); return 2; }
+        i = m - 1 - (s >> 1);
+        return 3;
+      }
+      if (s < 2 * m - 1) {
+        if (s & 1) { i = (s - 1) >> 1; return 1; }
+        i = s >> 1;
+        return 0;
+      }
+      const int t = s - (2 * m - 1);
+      if (t & 1) { i = m - 1 - ((t + 1) >> 1); return 3; }
+      i = m - 1 - (t >> 1);
+      return 2;
+    };
+    float* fb = a.f + b * a.fbs;
+    auto fptr = [&](int i) -> float* { return fb + (int64_t)i * a.fis; };
+    auto block = [&](int k, int i) -> const float* {
+      if (k == 1) return a.sub + b * a.lbs + (int64_t)i * a.lis;
+      if (k == 3) return a.sup + b * a.ubs + (int64_t)i * a.uis;
+      if (k == 0) return a.linv + b * a.dbs + (int64_t)i * a.dis;
+      return a.diag + b * a.dbs + (int64_t)i * a.dis;
+    };
+    // three register buffers: the block of stage s + 2 is requested while
+    // stage s computes (the chain is latency-bound: 20 workgroups)
+    float4 q0[NVMAX], q1[NVMAX], q2[NVMAX];
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    // branch-free and never consumed at load time: past the last stage the
+    // last block is read again, past nv a vector is read clamped and masked
+    // when used, and the f entry is read for every stage (used by SUB/SUP
+    // only), so the waits stay counted instead of draining the prefetches
+    auto fetch = [&](int s, float4 (&dst)[NVMAX], float& fv) __attribute__((always_inline)) {
+      int i;
+      const int k = decode(s < nst ? s : nst - 1, i);
+      const float4* row = reinterpret_cast<const float4*>(block(k, i) + (int64_t)rc * bs + cb);
+#pragma unroll
+      for (int v = 0; v < NVMAX; ++v) dst[v] = row[v < nv ? v : nv - 1];
+      fv = fptr(k == 1 ? i + 1 : i)[(int64_t)rc * ldf];
+    };
+    // LDS roles: X = input of the next DIAG stage, Y = last DIAG output
+    int X = 0, Y = 1;
+    if (tid < 3 * kInv1MaxBs) (&vb[0][0])[tid] = 0.f;  // entries past bs stay zero
+    if (tid + 256 < 3 * kInv1MaxBs) (&vb[0][0])[tid + 256] = 0.f;
+    __syncthreads();
+    if (mode == 2) {
+      if (tid < bs) vb[Y][tid] = fptr(m)[(int64_t)tid * ldf];  // the block past the end holds its solution
+    } else {
+      if (tid < bs) vb[X][tid] = fptr(0)[(int64_t)tid * ldf];
+    }
+    fetch(0, q0, f0);
+    fetch(1, q1, f1);
+    __syncthreads();
+    // the LDS roles travel by value (st = X | Y << 2): captured by
+    // reference they were kept in scratch, whose loads drain the prefetches
+    auto stage = [&](int s, const float4 (&cur)[NVMAX], float fcur, int st) __attribute__((always_inline)) -> int {
+      int X = st & 3, Y = st >> 2;
+      int i;
+      const int k = decode(s, i);
+      const int in = (k == 0) ? X : ((k == 2) ? ((mode == 0 && i == m - 1) ? Y : X) : Y);
+      const int o = 3 - X - Y;  // the buffer that is neither X nor Y
+      const float4* iv = reinterpret_cast<const float4*>(&vb[in][cb]);
+      float acc = 0.f;
+#pragma unroll
+      for (int v = 0; v < NVMAX; ++v) {
+        const float4 x = iv[v];
+        const float d = cur[v].x * x.x + cur[v].y * x.y + cur[v].z * x.z + cur[v].w * x.w;
+        acc += (v < nv) ? d : 0.f;
+      }
+      acc += __shfl_xor(acc, 1);
+      const float res = (k == 1 || k == 3) ? fcur - acc : acc;
+      if (h == 0 && r < bs) {
+        vb[o][r] = res;
+        if (k == 0 || k == 2) fptr(i)[(int64_t)r * ldf] = res;
+      }
+      // DIAG stages produce Y, SUB/SUP stages produce X
+      if (k == 0 || k == 2) Y = o;
+      else X = o;
+      __syncthreads();
+      return X | (Y << 2);
+    };
+    int st = X | (Y << 2);
+    for (int s = 0; s < nst; s += 3) {
+      fetch(s + 2, q2, f2);
+      st = stage(s, q0, f0, st);
+      if (s + 1 >= nst) break;
+      fetch(s + 3, q0, f0);
+      st = stage(s + 1, q1, f1, st);
+      if (s + 2 >= nst) break;
+      fetch(s + 4, q1, f1);
+      st = stage(s + 2, q2, f2, st);
+    }
+  }
+}
+
 template <typename T>
 hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
                       hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    // 16-byte vectors: every block offset and base a multiple of 4 floats
+    auto al = [](const ChainSet<T>& c) {
+      if (c.nchains == 0) return true;
+      const auto p4 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+      return c.linv != nullptr && p4(c.linv) && p4(c.diag) && p4(c.sub) && p4(c.sup) && c.dbs % 4 == 0 &&
+             c.dis % 4 == 0 && c.lbs % 4 == 0 && c.lis % 4 == 0 && c.ubs % 4 == 0 && c.uis % 4 == 0;
+    };
+    if (nrhs == 1 && c0.linv != nullptr && al(c0) && al(c1) && bs <= kInv1MaxBs && bs % 4 == 0) {
+      const int nt = c0.nchains + c1.nchains;
+      if (nt <= 0) return hipSuccess;
+      hipLaunchKernelGGL(k_btri_inv1, dim3(nt < 65535 ? nt : 65535), dim3(256), 0, s, bs,
+                         reinterpret_cast<const ChainSet<float>&>(c0), reinterpret_cast<const ChainSet<float>&>(c1),
+                         ldf, mode);
+      return hipGetLastError();
+    }
+  }
   int nr = nrhs >= 32 ? 32 : (nrhs >= 16 ? 16 : (nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1))));
   while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
   if (lu_solve_lds<T>(bs, nr) > kLdsMax || bs > (sizeof(T) == 8 ? 112 : 160)) return hipErrorInvalidValue;
