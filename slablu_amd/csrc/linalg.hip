@@ -912,11 +912,11 @@ static const size_t kPanelLdsBudget = 128 * 1024;
 
 // RP rows per thread in NT-thread register panels
 template <typename T, int W, int RP, int NT>
-hipError_t getrf_reg(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A, int* ipiv, int64_t pbs,
-                     int* info, hipStream_t s) {
+hipError_t getrf_reg(int batch, int nrows, int ncols, int kbeg, int kend, int npiv, View<T> A, int* ipiv,
+                     int64_t pbs, int* info, hipStream_t s) {
   hipError_t e;
-  for (int k0 = 0; k0 < nelim; k0 += W) {
-    const int w = (nelim - k0) < W ? (nelim - k0) : W;
+  for (int k0 = kbeg; k0 < kend; k0 += W) {
+    const int w = (kend - k0) < W ? (kend - k0) : W;
     hipLaunchKernelGGL((k_getf2_reg<T, W, RP, NT>), dim3(grid_batch(batch)), dim3(NT), 0, s, batch, nrows, k0, w,
                        npiv, A, ipiv, pbs, info);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -940,10 +940,40 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
   // panels held in registers, RP rows per thread: batched small matrices in
   // 256-thread workgroups (several per CU), up to 1024 rows in 512-thread
   // workgroups (two wavefronts per SIMD), four rows per thread up to 2048
-  if (nrows <= 256) return getrf_reg<T, 32, 1, 256>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 512) return getrf_reg<T, 32, 1, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 1024) return getrf_reg<T, 16, 2, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
-  if (nrows <= 2048) return getrf_reg<T, 8, 4, 512>(batch, nrows, ncols, nelim, npiv, A, ipiv, pbs, info, s);
+  // The panel shape follows the rows still below the diagonal: once they
+  // fit the next smaller shape, its wider panels take over (a 1000-row LU
+  // runs 31 16-column panels, then 16 32-column panels).
+  if (nrows <= 2048) {
+    hipError_t e;
+    int k0 = 0;
+    while (k0 < nelim) {
+      const int R = nrows - k0;
+      // first column from which the remaining rows fit `cap`, on this
+      // phase's panel grid (multiples of W from k0), capped at nelim
+      auto until = [&](int cap, int W) {
+        const int need = R - cap;  // columns to eliminate before R <= cap
+        const int c = k0 + ((need + W - 1) / W) * W;
+        return c < nelim ? c : nelim;
+      };
+      int k1;
+      if (R <= 256) {
+        k1 = nelim;
+        e = getrf_reg<T, 32, 1, 256>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
+      } else if (R <= 512) {
+        k1 = until(256, 32);
+        e = getrf_reg<T, 32, 1, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
+      } else if (R <= 1024) {
+        k1 = until(512, 16);
+        e = getrf_reg<T, 16, 2, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
+      } else {
+        k1 = until(1024, 8);
+        e = getrf_reg<T, 8, 4, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
+      }
+      if (e != hipSuccess) return e;
+      k0 = k1;
+    }
+    return hipSuccess;
+  }
   // taller panels: LDS-resident panel, generic interchanges
   int nb = 32;
   while (nb > 4 && (size_t)nrows * nb * sizeof(T) > kPanelLdsBudget) nb >>= 1;
