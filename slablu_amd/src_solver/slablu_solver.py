@@ -665,10 +665,18 @@ class Domain_Solver:
             pack(rI, rF, V[:, 0])
             beta = torch.linalg.norm(V[:, 0], dim=1)
             V[:, 0] /= beta.clamp_min(1e-300)[:, None]
-            beta = beta.cpu().numpy()
-            H = np.zeros((nc, kmax + 1, kmax))
             xc0 = x_face[Ic].T.contiguous()                       # (nc, |I_Ctot|)
-            fixed2 = (torch.linalg.norm(x_face) ** 2 - torch.linalg.norm(xc0) ** 2).clamp_min(0)
+            # one transfer: beta and |x_face|^2
+            bx = torch.cat((beta, (torch.linalg.norm(x_face) ** 2).reshape(1))).cpu().numpy()
+            beta, xf2 = bx[:nc], float(bx[nc])
+            H = np.zeros((nc, kmax + 1, kmax))
+            # face parts of the directions: Gram matrix G and g = Z_face^T xc0,
+            # so the face change and the face norm of every iterate follow on
+            # the host from y (one device-to-host transfer per step):
+            #   |sum dy_i Z_i|^2 = dy^T G dy,
+            #   |x_face + sum y_i Z_i|^2 = |x_face|^2 + 2 y^T g + y^T G y
+            G = np.zeros((nc, kmax, kmax))
+            gx = np.zeros((nc, kmax))
             y_prev = np.zeros((nc, 0))
             for j in range(kmax):
                 zI, zC = self._correction(*unpack(V[:, j]))
@@ -685,17 +693,26 @@ class Domain_Solver:
                         h[col, :j + 1] += c
                 h[:, j + 1] = torch.linalg.norm(w, dim=1)
                 V[:, j + 1] = w / h[:, j + 1].clamp_min(1e-300)[:, None]
-                H[:, :j + 2, j] = h.cpu().numpy()
+                gr = torch.empty(nc, j + 2, dtype=torch.float64, device=dev)
+                for col in range(nc):
+                    zf = Z[col, j, nI:]
+                    torch.mv(Z[col, :j + 1, nI:], zf, out=gr[col, :j + 1])
+                    gr[col, j + 1] = torch.dot(zf, xc0[col])
+                hg = torch.cat((h, gr), 1).cpu().numpy()
+                H[:, :j + 2, j] = hg[:, :j + 2]
+                G[:, :j + 1, j] = hg[:, j + 2:2 * j + 3]
+                G[:, j, :j + 1] = hg[:, j + 2:2 * j + 3]
+                gx[:, j] = hg[:, 2 * j + 3]
                 y = np.zeros((nc, j + 1))
                 for col in range(nc):
                     rhs = np.zeros(j + 2)
                     rhs[0] = beta[col]
                     y[col] = np.linalg.lstsq(H[col, :j + 2, :j + 1], rhs, rcond=None)[0]
                 dy = y - np.pad(y_prev, ((0, 0), (0, 1)))
-                yt = torch.from_numpy(np.stack((y, dy), 1)).to(dev)           # (nc, 2, j+1)
-                comb = (yt.unsqueeze(-1) * Z[:, None, :j + 1, nI:]).sum(2)      # (nc, 2, |I_Ctot|)
-                xnorm = torch.sqrt(fixed2 + torch.linalg.norm(xc0 + comb[:, 0]) ** 2)
-                rel = (torch.linalg.norm(comb[:, 1]) / xnorm).item()
+                Gj = G[:, :j + 1, :j + 1]
+                num2 = float(np.einsum("ci,cij,cj->", dy, Gj, dy))
+                xn2 = xf2 + float(np.einsum("ci,cij,cj->", y, Gj, y)) + 2.0 * float((y * gx[:, :j + 1]).sum())
+                rel = float(np.sqrt(max(num2, 0.0)) / np.sqrt(max(xn2, 1e-300)))
                 self.refine_history.append(rel)
                 y_prev = y
                 steps += 1
