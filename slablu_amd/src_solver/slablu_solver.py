@@ -87,6 +87,13 @@ class Domain_Solver:
         self.refine_method, self.refine_restart = refine_method, refine_restart
         # synchronize inside build to split stage timings (bench/profiling only)
         self.stage_sync = True
+        # from the second solve with one factorization on, FGMRES replays its
+        # preconditioned operator step (FP32 correction + FP64 residual of
+        # the full system) as a captured HIP graph; the first solve stays
+        # eager, since capturing costs about two steps
+        self.refine_graphs = True
+        self._gsteps = {}
+        self._solves_since_build = 0
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.hps_disc(box_geom, h, p, pdo_op)
         self.hps_panel_split()
@@ -162,6 +169,8 @@ class Domain_Solver:
             raise NotImplementedError("solver_type %r: only 'slabLU' is provided" % (solver_type,))
         self.sparse_assembly = sparse_assembly
         self.solver_type = "slabLU"
+        self._gsteps = {}  # captured steps read the factors of the previous build
+        self._solves_since_build = 0
         self.wdt = torch.float32 if self.precision == "mixed" else torch.float64
         t = {}
         sync = torch.cuda.synchronize
@@ -623,6 +632,33 @@ class Domain_Solver:
                 break
             prev = rel_corr if it >= 1 else None
 
+    def _graph_step(self, nc):
+        """The FGMRES operator step (rI, rF) -> (zI, zC, wI, wF) = (M^{-1} r,
+        A M^{-1} r) captured once per column count as a HIP graph: one launch
+        instead of a few hundred small ones per Krylov step.  Returns the
+        graph and its static buffers."""
+        g = self._gsteps.get(nc)
+        if g is not None:
+            return g
+        hps, ops = self.hps, self.hps.ops
+        rI = torch.zeros(hps.nboxes, ops.ni, nc, dtype=torch.float64, device=self.device)
+        rF = torch.zeros(self._Ic().shape[0], nc, dtype=torch.float64, device=self.device)
+        cur = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):                   # warm-up outside the capture
+            for _ in range(2):
+                zI, zC = self._correction(rI, rF)
+                self._apply_full(zI, zC)
+        cur.wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            zI, zC = self._correction(rI, rF)
+            wI, wF = self._apply_full(zI, zC)
+        g = (graph, rI, rF, zI, zC, wI, wF)
+        self._gsteps[nc] = g
+        return g
+
     def _refine_fgmres(self, x_face, uc, fb=None):
         """Flexible GMRES on the full system, right-preconditioned by the FP32
         two-level solve M: the Arnoldi relation A Z_j = V_{j+1} H_j holds in
@@ -655,6 +691,8 @@ class Domain_Solver:
             aC.copy_(v[:, nI:].T)
             return aI, aC
 
+        use_graph = self.refine_graphs and (self._solves_since_build > 0 or nc in self._gsteps)
+        gstep = self._graph_step(nc) if use_graph else None
         steps = 0
         done = False
         prev = None
@@ -679,9 +717,15 @@ class Domain_Solver:
             gx = np.zeros((nc, kmax))
             y_prev = np.zeros((nc, 0))
             for j in range(kmax):
-                zI, zC = self._correction(*unpack(V[:, j]))
+                if gstep is not None:
+                    graph, gI, gF, zI, zC, wI, wF = gstep
+                    gI.view(nI, nc).copy_(V[:, j, :nI].T)
+                    gF.copy_(V[:, j, nI:].T)
+                    graph.replay()
+                else:
+                    zI, zC = self._correction(*unpack(V[:, j]))
+                    wI, wF = self._apply_full(zI, zC)
                 pack(zI, zC, Z[:, j])
-                wI, wF = self._apply_full(zI, zC)
                 w = torch.empty(nc, ntot, dtype=torch.float64, device=dev)
                 pack(wI, wF, w)
                 h = torch.zeros(nc, j + 2, dtype=torch.float64, device=dev)
@@ -750,6 +794,7 @@ class Domain_Solver:
         self.refine_history = []
         if self.refine_method == "fgmres":
             self._refine_fgmres(x_face, uc, fb=fb)
+            self._solves_since_build += 1
         else:
             self._refine_richardson(x_face, uc, fb=fb)
         uu = torch.empty(hps.nboxes, ops.p * ops.p, nc, dtype=torch.float64, device=dev)

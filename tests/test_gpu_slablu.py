@@ -132,6 +132,33 @@ def test_mixed_refined_vs_fp64_oracle():
     assert abs(true_err - r["true_err"]) < 1e-9
 
 
+def test_refinement_graph_replays_match_eager():
+    """The FGMRES operator step replayed as a captured HIP graph gives the
+    eager step's iterates: two solves with graphs (replays with eager work and
+    synchronizations between them, the second reusing the captured graph)
+    against one without, solution and refinement history identical."""
+    n, p, ppw = 200, 22, 20
+    dom, kh, a, uu_dir = _solver(n, p, ppw, precision="mixed")
+    dom.build(verbose=False)
+    dom.refine_graphs = False
+    sol0 = dom.solve(uu_dir)[0]
+    hist0 = list(dom.refine_history)
+    dom.refine_graphs = True
+    for _ in range(2):
+        sol1 = dom.solve(uu_dir)[0]       # a solve after the first: captured step
+        torch.cuda.synchronize()
+        assert len(dom._gsteps) == 1
+        assert dom.refine_history == hist0
+        assert torch.equal(sol1, sol0)
+    # a rebuild drops the captured step (it reads the old factors), and the
+    # first solve after it runs eagerly
+    dom.build(verbose=False)
+    assert dom._gsteps == {}
+    sol2 = dom.solve(uu_dir)[0]
+    assert dom._gsteps == {}
+    assert torch.equal(sol2, sol0)
+
+
 def _periodic(n, p, ppw, precision, bufc=0.6, m=2):
     """x-periodic Helmholtz on the unit square (n/(p-2) leaves per side, so
     the leaves tile the period exactly), Dirichlet data on the bottom and top
