@@ -19,8 +19,8 @@ namespace {
 // is loaded at hand-over), its right-hand-side rows and its pivots are
 // already being loaded into registers, so global latency is off the
 // critical path.  Workgroups past c0's chains take c1's chains.
-template <typename T, int NR, int PF>
-__global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, ChainSet<T> c1, int64_t ldf,
+template <typename T, int NR, int PF, bool INV>
+__global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solve(int bs, ChainSet<T> c0, ChainSet<T> c1, int64_t ldf,
                                                        int nrhs, int mode) {
   // rows per thread in the block products, for the largest block the fused
   // factorization accepts (112 rows in FP64, 160 in FP32)
@@ -70,7 +70,8 @@ __global__ __launch_bounds__(256) void k_btri_lu_solve(int bs, ChainSet<T> c0, C
     };
     // inverse factors: DIAGL multiplies by L^{-1} P (linv), DIAGU by U^{-1}
     // (diag), both plain block products
-    const bool inv = a.linv != nullptr;
+    // (compile-time, so each variant keeps only the registers its stages need)
+    constexpr bool inv = INV;
     // the last block's DIAGU of a full sweep reuses the resident DIAGL block
     auto resident = [&](int k, int i) -> bool { return !inv && mode == 0 && k == 2 && i == m - 1; };
     T* fb = a.f + b * a.fbs;
@@ -237,13 +238,18 @@ template <typename T, int NR, int PF>
 hipError_t launch_lu_solve(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
                            hipStream_t s) {
   const size_t lds = lu_solve_lds<T>(bs, NR);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const bool inv = c0.linv != nullptr;
+  const void* fn = inv ? reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF, true>)
+                       : reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF, false>);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const int nt = c0.nchains + c1.nchains;
   if (nt <= 0) return hipSuccess;
   dim3 grid(nt < 65535 ? nt : 65535, (nrhs + NR - 1) / NR);
-  hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
+  if (inv)
+    hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF, true>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
+  else
+    hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF, false>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
   return hipGetLastError();
 }
 
@@ -254,7 +260,11 @@ template <typename T, int NR>
 hipError_t launch_lu_solve_pf(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
                               hipStream_t s) {
   constexpr int PFBIG = sizeof(T) == 8 ? (NR >= 32 ? 8 : (NR >= 16 ? 16 : 40)) : 40;
-  if ((int64_t)bs * bs <= 256 * 8) return launch_lu_solve<T, NR, 8>(bs, c0, c1, ldf, nrhs, mode, s);
+  // inverse factors with 32 columns: a small prefetch keeps two workgroups
+  // per CU (their LDS fits twice), which hides more than a whole-block
+  // prefetch in one
+  if ((int64_t)bs * bs <= 256 * 8 || (NR == 32 && c0.linv != nullptr))
+    return launch_lu_solve<T, NR, 8>(bs, c0, c1, ldf, nrhs, mode, s);
   return launch_lu_solve<T, NR, PFBIG>(bs, c0, c1, ldf, nrhs, mode, s);
 }
 
