@@ -409,6 +409,9 @@ hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int6
     }
   }
   int nr = nrhs >= 32 ? 32 : (nrhs >= 16 ? 16 : (nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1))));
+  // small blocks in substitution form: 8-column tiles (about 130 VGPRs, three
+  // workgroups per CU) beat 32-column ones (one workgroup per CU)
+  if (bs <= 32 && c0.linv == nullptr && nr > 8) nr = 8;
   while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
   if (lu_solve_lds<T>(bs, nr) > kLdsMax || bs > (sizeof(T) == 8 ? 112 : 160)) return hipErrorInvalidValue;
   switch (nr) {
