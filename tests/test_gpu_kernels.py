@@ -175,6 +175,23 @@ def test_getrf_square_matches_lapack(dtype, n):
         assert _lu_check(A0[k].to(dtype).double().numpy(), LU[k], pv[k], n) < 10 * TOL[dtype] * max(1, n / 10)
 
 
+@pytest.mark.parametrize("n", [1100, 1500])
+def test_getrf_tall_panel_phases(n):
+    """1025-2048 rows start with 8-column panels of four rows per thread and
+    switch to wider panels as the rows below the diagonal shrink (16, then
+    32 columns); pivots as LAPACK's, factors to rounding."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(n + 1)
+    A0 = torch.randn(1, n, n, generator=g, dtype=torch.float64)
+    A = A0.to("cuda").contiguous()
+    ipiv = torch.zeros(1, n, dtype=torch.int32, device="cuda")
+    hip.getrf(A, ipiv)
+    torch.cuda.synchronize()
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A0[0].numpy())
+    assert np.array_equal(ipiv.cpu().numpy()[0] - 1, piv_ref)
+    assert np.abs(A.cpu().numpy()[0] - lu_ref).max() < 1e-9 * max(1, np.abs(lu_ref).max())
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_getrf_rectangular_and_bordered(dtype):
     """[A11 A12; A21 A22] with pivots restricted to the first ni rows:
