@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_getrs_wave(int batch, int n, int nrhs, 
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   T* X = reinterpret_cast<T*>(smem_raw) + (size_t)wave * n * NC;  // n x NC
-  const int b = blockIdx.x * 4 + wave;
+  const int b = blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (b >= batch) return;
   constexpr int V = 16 / sizeof(T);
   const T* base = LU.at(b, 0, 0);
@@ -304,14 +304,15 @@ hipError_t getrs_perm(int batch, int n, int nrhs, View<T> LU, const int* perm, i
                       hipStream_t s) {
   if (nrhs > 8 || nrhs < 1 || n < 1) return hipErrorInvalidValue;
   const int nc = nrhs == 1 ? 1 : (nrhs <= 2 ? 2 : (nrhs <= 4 ? 4 : 8));
-  const size_t lds = 4 * (size_t)n * nc * sizeof(T);
+  // one wavefront (one matrix) per workgroup: the dispatcher spreads the
+  // matrices over the CUs one at a time instead of four at a time
+  const size_t lds = (size_t)n * nc * sizeof(T);
   if (lds > kLdsMax) return hipErrorInvalidValue;
   auto launch = [&](auto kern) -> hipError_t {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    const int nwg = (batch + 3) / 4;
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), lds, s, batch, n, nrhs, LU, perm, qbs, B);
+    hipLaunchKernelGGL(kern, dim3(batch), dim3(64), lds, s, batch, n, nrhs, LU, perm, qbs, B);
     return hipGetLastError();
   };
   if (nc == 1) return launch(&k_getrs_wave<T, 1>);
