@@ -55,9 +55,66 @@ __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nr
   const int r0 = blockIdx.y * rows_per_block;
   const int p2 = p * p;
   T* Mb = M + (int64_t)b * mbs;
+  // the nodes of this thread's columns, once for all rows of the block
+  constexpr int CPT = 4;
+  int jj1[CPT], jj2[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int col = threadIdx.x + 256 * k;
+    const int cc = col < ncols ? col : ncols - 1;
+    if (xnodes && cc >= ni) {
+      const int node = xnodes[cc - ni];
+      jj1[k] = node / p;
+      jj2[k] = node % p;
+    } else {
+      leaf_col_node(cc, ni, f, p, jj1[k], jj2[k]);
+    }
+  }
+  const bool tabled = ncols <= 256 * CPT && blockDim.x == 256 && rows_per_block <= 64;
+  // the coefficients of the block's rows, loaded once (one round trip
+  // instead of one per row)
+  __shared__ double cf[5][64];
+  if (tabled && c12 == nullptr) {
+    const int t = threadIdx.x;
+    if (t < rows_per_block) {
+      const int r = r0 + t;
+      const int rc = r < ni ? r : ni - 1;
+      const int64_t ci = (int64_t)b * p2 + (1 + rc / f) * p + 1 + rc % f;
+      cf[0][t] = -c11[ci];
+      cf[1][t] = -c22[ci];
+      cf[2][t] = c1 ? c1[ci] : 0.0;
+      cf[3][t] = c2 ? c2[ci] : 0.0;
+      cf[4][t] = c ? c[ci] : 0.0;
+    }
+    __syncthreads();
+  }
   for (int rr = 0; rr < rows_per_block; ++rr) {
     const int r = r0 + rr;
     if (r >= nrows) break;
+    if (r < ni && tabled && c12 == nullptr) {
+      // without the cross term a row couples only to its node's grid lines
+      // (j1 == i1 or j2 == i2); every other entry is an exact zero, written
+      // without evaluating the terms (same value as the full sum: +0)
+      const int i1 = 1 + r / f, i2 = 1 + r % f;
+      const double v11 = cf[0][rr], v22 = cf[1][rr], v1 = cf[2][rr], v2 = cf[3][rr], v0 = cf[4][rr];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int col = threadIdx.x + 256 * k;
+        if (col < ncols) {
+          const int j1 = jj1[k], j2 = jj2[k];
+          double val = 0.0;
+          if (i1 == j1 || i2 == j2) {
+            val += v11 * (i2 == j2 ? Dsq[i1 * p + j1] : 0.0);
+            val += v22 * (i1 == j1 ? Dsq[i2 * p + j2] : 0.0);
+            if (c1) val += v1 * (-(i2 == j2 ? D[i1 * p + j1] : 0.0));
+            if (c2) val += v2 * (-(i1 == j1 ? D[i2 * p + j2] : 0.0));
+            if (c) val += v0 * ((i1 == j1 && i2 == j2) ? 1.0 : 0.0);
+          }
+          Mb[(int64_t)r * ldm + col] = (T)val;
+        }
+      }
+      continue;
+    }
     if (r < ni) {
       const int i1 = 1 + r / f, i2 = 1 + r % f;
       const int node = i1 * p + i2;
@@ -100,7 +157,7 @@ hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const d
                          int nxc) {
   const int f = p - 2, ni = f * f, nx = 4 * f;
   const int nrows = ni + (border ? nx : 0);
-  const int rpb = 8;
+  const int rpb = 32;
   dim3 grid(nboxes, (nrows + rpb - 1) / rpb);
   hipLaunchKernelGGL(k_leaf_assemble<T>, grid, dim3(256), 0, s, nboxes, p, nrows, rpb, D, Dsq, c11, c22, c12, c1,
                      c2, c, Nxtot, M, ldm, mbs, xnodes, nxc);
