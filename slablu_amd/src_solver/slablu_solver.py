@@ -85,7 +85,8 @@ class Domain_Solver:
             raise ValueError("refine_method must be 'fgmres' or 'richardson'")
         self.refine_tol, self.refine_maxit = refine_tol, refine_maxit
         self.refine_method, self.refine_restart = refine_method, refine_restart
-        # synchronize inside build to split stage timings (bench/profiling only)
+        # stage timings of build() from events on the stream (no host
+        # synchronization between stages)
         self.stage_sync = True
         # from the second solve with one factorization on, FGMRES replays its
         # preconditioned operator step (FP32 correction + FP64 residual of
@@ -173,29 +174,40 @@ class Domain_Solver:
         self._solves_since_build = 0
         self.wdt = torch.float32 if self.precision == "mixed" else torch.float64
         t = {}
-        sync = torch.cuda.synchronize
-        sync()
+        # stage boundaries are events on the stream; the host never waits in
+        # between, so no stage starts behind a drained queue
+        ev = {}
+
+        def mark(name):
+            if self.stage_sync:
+                ev[name] = torch.cuda.Event(enable_timing=True)
+                ev[name].record()
+
+        self._mark = mark
+        torch.cuda.synchronize()
         tic0 = time.time()
+        mark("start")
         self.hps.factor_leaves(self.device, self.wdt)
-        sync()
-        t["toc_DtN"] = time.time() - tic0
+        mark("dtn")
         self.Dt7 = self._dt7(self.hps.leaf["DtN"])
-        tic = time.time()
         self.build_Aloc_solver()
-        sync()
-        t["toc_Aloc"] = time.time() - tic
-        tic = time.time()
+        mark("aloc")
         self.construct_Ared()
-        sync()
-        t["toc_Ared"] = time.time() - tic
-        tic = time.time()
-        self._t_tred0 = tic
-        self.t_Tform = float("nan")
+        mark("ared")
         self.build_Tred_solver()
-        sync()
-        t["toc_Tred"] = time.time() - tic
-        t["toc_Tform"] = self.t_Tform
+        mark("tred")
+        torch.cuda.synchronize()
         toc_build = time.time() - tic0
+        if self.stage_sync:
+            el = lambda a, b: ev[a].elapsed_time(ev[b]) / 1e3
+            t["toc_DtN"] = el("start", "dtn")
+            t["toc_Aloc"] = el("dtn", "aloc")
+            t["toc_Ared"] = el("aloc", "ared")
+            t["toc_Tred"] = el("ared", "tred")
+            t["toc_Tform"] = el("ared", "tform") if "tform" in ev else float("nan")
+        else:
+            t.update(toc_DtN=float("nan"), toc_Aloc=float("nan"), toc_Ared=float("nan"), toc_Tred=float("nan"),
+                     toc_Tform=float("nan"))
         mem = (self.hps.leaf["M"].numel() * self.hps.leaf["M"].element_size() + self.Aloc_solver.nbytes +
                self.Ared_solver.nbytes + self.Tred_solver.nbytes) / 1e9
         info = dict(toc_build=toc_build, mem_build=mem, toc_assembly=t["toc_DtN"], **t)
@@ -368,9 +380,8 @@ class Domain_Solver:
         K6[:, ja, :, :, ja, :] += Q.view(Np, n1, 2 * f, 2, f).permute(1, 0, 2, 3, 4)
         Kf = K.view(Np, n1, 2, f, 2 * L).permute(0, 2, 1, 3, 4).reshape(Np, 2 * L, 2 * L)
         del upad, K
-        if self.stage_sync:
-            torch.cuda.synchronize()
-            self.t_Tform = time.time() - self._t_tred0
+        if getattr(self, "_mark", None) is not None:
+            self._mark("tform")
         Aii = self.interface_blocks()
         KR = Kf[:, L:, L:].to(torch.float64)
         KL = Kf[:, :L, :L].to(torch.float64)
