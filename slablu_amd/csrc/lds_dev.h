@@ -78,16 +78,16 @@ __device__ void mm_global_global(const T* A, const T* B, int bs, T* TA, T* TB, T
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int bm1 = bs - 1;
   T va[NE], vb[NE];
+  // clamped addresses, masks applied when the chunk is written to LDS: the
+  // next chunk's loads stay in flight across the current chunk's products
   auto load = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
       const int e = tid + 256 * u;
       const int ra = e / KC, ka = k0 + e % KC;
-      const T a = A[(int64_t)(ra < bm1 ? ra : bm1) * bs + (ka < bm1 ? ka : bm1)];
-      va[u] = (ra < bs && ka < bs) ? a : T(0);
+      va[u] = A[(int64_t)(ra < bm1 ? ra : bm1) * bs + (ka < bm1 ? ka : bm1)];
       const int kb = k0 + e / BP, cb = e % BP;
-      const T v = B[(int64_t)(kb < bm1 ? kb : bm1) * bs + (cb < bm1 ? cb : bm1)];
-      vb[u] = (kb < bs && cb < bs) ? v : T(0);
+      vb[u] = B[(int64_t)(kb < bm1 ? kb : bm1) * bs + (cb < bm1 ? cb : bm1)];
     }
   };
   load(0);
@@ -96,8 +96,9 @@ __device__ void mm_global_global(const T* A, const T* B, int bs, T* TA, T* TB, T
     for (int u = 0; u < NE; ++u) {
       const int e = tid + 256 * u;
       if (e < BP * KC) {
-        TA[(e / KC) * (KC + 1) + e % KC] = va[u];
-        TB[(e / BP) * (BP + 1) + e % BP] = vb[u];
+        const int ra = e / KC, ka = k0 + e % KC, kb = k0 + e / BP, cb = e % BP;
+        TA[(e / KC) * (KC + 1) + e % KC] = ((ra < bs) & (ka < bs)) ? va[u] : T(0);
+        TB[(e / BP) * (BP + 1) + e % BP] = ((kb < bs) & (cb < bs)) ? vb[u] : T(0);
       }
     }
     __syncthreads();
@@ -460,7 +461,7 @@ __device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* inf
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
           const T v = S[(k0 + rc) * ld + k0 + c];
-          x[s][c] = (rr < R && c < nb) ? v : T(0);
+          x[s][c] = ((rr < R) & (c < nb)) ? v : T(0);
         }
       }
       static_for<0, NB>([&](auto jc) __attribute__((always_inline)) {
@@ -590,13 +591,13 @@ __device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* inf
           for (int a = 0; a < RT; ++a) {
             const int r = ty + 16 * a;
             const T v = S[r * ld + k0 + j];
-            av[a] = (r >= kk && r < n) ? v : T(0);
+            av[a] = ((r >= kk) & (r < n)) ? v : T(0);
           }
 #pragma unroll
           for (int q = 0; q < RT; ++q) {
             const int c = tx + 16 * q;
             const T v = S[(k0 + j) * ld + c];
-            bv[q] = (c >= kk && c < n) ? v : T(0);
+            bv[q] = ((c >= kk) & (c < n)) ? v : T(0);
           }
 #pragma unroll
           for (int a = 0; a < RT; ++a)
