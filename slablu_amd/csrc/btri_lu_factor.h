@@ -9,6 +9,26 @@ namespace slab {
 
 namespace {
 
+// Elements 0..n-1 from global memory into LDS, eight loads in flight per
+// thread: load(e) reads element e (e < n; the tail reads element n-1
+// again), store(e, v) writes it.
+template <typename T, typename Load, typename Store>
+__device__ inline void gather8(int n, Load load, Store store) {
+  for (int e0 = threadIdx.x; e0 < n; e0 += 256 * 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      v[u] = load(e < n ? e : n - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      if (e < n) store(e, v[u]);
+    }
+  }
+}
+
 // One workgroup per chain for the whole factorization.  Per block: stage
 // D_i, subtract Lfac_{i-1} Ufac_{i-1} (products from global in KC-wide LDS
 // chunks), blocked LU with partial pivoting in LDS, then Ufac_i (rows gathered
@@ -71,10 +91,13 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int bs, ChainSet<T> c0, 
           const int nc = (bs - c0) < XC ? (bs - c0) : XC;
           const float rnc = 1.0f / (float)nc;
           // Ufac chunk: L^{-1} (P sup_i)[:, c0:c0+nc]
-          for (int e = tid; e < bs * nc; e += 256) {
+          gather8<T>(bs * nc, [&](int e) {
             const int r = idiv(e, rnc), c = e - r * nc;
-            X[r * ldx + c] = Ui[(int64_t)Q[r] * bs + c0 + c];
-          }
+            return Ui[(int64_t)Q[r] * bs + c0 + c];
+          }, [&](int e, T v) {
+            const int r = idiv(e, rnc), c = e - r * nc;
+            X[r * ldx + c] = v;
+          });
           __syncthreads();
           tri_solve_wide<T, false, true>(Lat, bs, X, ldx, nc);
           for (int e = tid; e < bs * nc; e += 256) {
@@ -83,10 +106,13 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int bs, ChainSet<T> c0, 
           }
           __syncthreads();
           // Lfac chunk: rows c0..c0+nc of sub_i U^{-1}, i.e. U^T Z = sub_i^T
-          for (int e = tid; e < bs * nc; e += 256) {
+          gather8<T>(bs * nc, [&](int e) {
             const int c = idiv(e, rbs), r = e - c * bs;
-            X[r * ldx + c] = Li[(int64_t)(c0 + c) * bs + r];
-          }
+            return Li[(int64_t)(c0 + c) * bs + r];
+          }, [&](int e, T v) {
+            const int c = idiv(e, rbs), r = e - c * bs;
+            X[r * ldx + c] = v;
+          });
           __syncthreads();
           tri_solve_wide<T, false, false>(UTat, bs, X, ldx, nc);
           for (int e = tid; e < bs * nc; e += 256) {
