@@ -125,8 +125,8 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           const int e = tid + 256 * u, r = e >> lgnr, c = e & (nr - 1);
-          const T v = *(frow(fi, r < bs ? r : bs - 1) + (c < cm1 ? c : cm1));
-          pfx[u] = (e < nf && c < ncols) ? v : T(0);
+          // masked when committed, so the loads stay in flight meanwhile
+          pfx[u] = *(frow(fi, r < bs ? r : bs - 1) + (c < cm1 ? c : cm1));
         }
       }
       if (k == 0 && !inv) {
@@ -148,13 +148,26 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
           if (e < nn) Mt[e + idiv(e, rbs)] = pf[u];
         }
         const T* src = block_src(k, i);
-        for (int e = tid + 256 * PF; e < nn; e += 256) Mt[e + idiv(e, rbs)] = src[e];
+        // the rest of the block, eight loads in flight per thread
+        for (int e0 = tid + 256 * PF; e0 < nn; e0 += 256 * 8) {
+          T v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int e = e0 + 256 * u;
+            v[u] = src[e < nn ? e : nn - 1];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int e = e0 + 256 * u;
+            if (e < nn) Mt[e + idiv(e, rbs)] = v[u];
+          }
+        }
       }
       if (f_row(k, i) >= 0) {
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           const int e = tid + 256 * u, r = e >> lgnr, c = e & (nr - 1);
-          if (e < nf) Xc[r * ldx + c] = pfx[u];
+          if (e < nf) Xc[r * ldx + c] = (c < ncols) ? pfx[u] : T(0);
         }
       }
       if (k == 0 && !inv) {

@@ -706,7 +706,7 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
           const T v = stage[wave][lane][c];
-          x[sl][c0 + c] = (!dn[sl] && c0 + c < w) ? v : T(0);
+          x[sl][c0 + c] = (!dn[sl] & (c0 + c < w)) ? v : T(0);
         }
         __syncthreads();
       });
