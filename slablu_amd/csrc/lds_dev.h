@@ -702,12 +702,12 @@ __device__ inline void sub_mat_mfma32(const T* Mt, const T* Y, T* Xc, int bs, in
     const int kc = kin ? k : bm1;
     const T yv = Y[kc * ldx + ycol];
     const T b = kin ? yv : T(0);
+    // every tile's load is issued before the products (one LDS round trip
+    // per k step); tiles past the block multiply zeros and are not stored
 #pragma unroll
     for (int j = 0; j < MAXT; ++j) {
-      if (rt0 + 2 * j < nrt) {  // uniform in the wavefront
-        const T av = Mt[aoff[j] + kc];
-        acc[j] = Mfma<T>::mma(kin ? av : T(0), b, acc[j]);
-      }
+      const T av = Mt[aoff[j] + kc];
+      acc[j] = Mfma<T>::mma((kin & (rt0 + 2 * j < nrt)) ? av : T(0), b, acc[j]);
     }
   }
   if constexpr (SET) __syncthreads();
