@@ -76,12 +76,12 @@ __device__ void gj_inverse(T* S, int n, int* piv, T* red_v, int* red_i, int* s_p
 #pragma unroll
     for (int q = 0; q < RT; ++q) {
       const int c = tx + 16 * q;
-      rowk[q] = (c < n && c != k) ? S[k * ld + c] * rp : T(0);
+      rowk[q] = ((c < n) & (c != k)) ? S[k * ld + c] * rp : T(0);
     }
 #pragma unroll
     for (int a = 0; a < RT; ++a) {
       const int r = ty + 16 * a;
-      colk[a] = (r < n && r != k) ? S[r * ld + k] : T(0);
+      colk[a] = ((r < n) & (r != k)) ? S[r * ld + k] : T(0);
     }
 #pragma unroll
     for (int a = 0; a < RT; ++a) {

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_getrs_small(int batch, int n, int nrhs,
           for (int u = 0; u < 8; ++u) {
             const int e = e0 + 256 * u, r = e >> 6, c = e & 63;
             const T w = *LU.at(b, d0 + (r < tb ? r : tb - 1), d0 + (c < tb ? c : tb - 1));
-            v[u] = (r < tb && c < tb) ? w : T(0);
+            v[u] = ((r < tb) & (c < tb)) ? w : T(0);
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_getrs_small(int batch, int n, int nrhs,
           for (int u = 0; u < 8; ++u) {
             const int r = rb + u;
             const T v = *LU.at(b, r < r1 ? r : r1 - 1, d0 + (act ? lane : 0));
-            l[u] = (r < r1 && act) ? v : T(0);
+            l[u] = ((r < r1) & act) ? v : T(0);
           }
           for (int c = 0; c < nrhs; ++c) {
             const T xv = X[(d0 + rl) * nrhs + c];

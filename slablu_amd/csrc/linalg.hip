@@ -388,7 +388,7 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int batch, int n, int d0, int 
     for (int u = 0; u < TB * TB / 64; ++u) {
       const int e = tid + 64 * u, r = e / TB, c = e % TB;
       const T v = *A.at(b, d0 + (r < tm1 ? r : tm1), d0 + (c < tm1 ? c : tm1));
-      Ls[r][c] = (r < tb && c < tb) ? v : ((r == c) ? T(1) : T(0));
+      Ls[r][c] = ((r < tb) & (c < tb)) ? v : ((r == c) ? T(1) : T(0));
     }
     __syncthreads();
     T x[TB];
@@ -845,7 +845,7 @@ __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, 
 #pragma unroll
       for (int c = 0; c < W; ++c) {
         const T v = *A.at(b, k0 + (r < wm1 ? r : wm1), k0 + (c < wm1 ? c : wm1));
-        L[r][c] = (r < w && c < r) ? v : T(0);
+        L[r][c] = ((r < w) & (c < r)) ? v : T(0);
       }
     }
     __syncthreads();
@@ -1096,7 +1096,7 @@ __global__ __launch_bounds__(256) void k_getri_prep(int n, int np, int nb, View<
   for (int i = ty; i < 32; i += 8) {
     const int r = r0 + i, c = c0 + tx;
     const T v = *LU.at(b, r < nm1 ? r : nm1, c < nm1 ? c : nm1);
-    const T w = (r < n && c < n) ? v : T(0);
+    const T w = ((r < n) & (c < n)) ? v : T(0);
     t[i][tx] = w;
     W0[(int64_t)r * np + c] = w;
   }
@@ -1116,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_getri_diag(int n, int np, int nb, const
   for (int e = tid; e < 32 * 32; e += 256) {
     const int r = e >> 5, c = e & 31;
     const T v = A[(int64_t)(d0 + r) * np + d0 + c];
-    Ls[r][c] = (d0 + r < n && d0 + c < n) ? v : ((r == c) ? T(1) : T(0));
+    Ls[r][c] = ((d0 + r < n) & (d0 + c < n)) ? v : ((r == c) ? T(1) : T(0));
   }
   __syncthreads();
   if (tid < 32) {
