@@ -566,13 +566,25 @@ __device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* inf
           }
         }
       }
+      // destination rows read up front (one LDS round trip), -1 = no store
+      int ru[NB], rd[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int r = s_row[q];
+        ru[q] = (q < nb) ? r : -1;
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const int r = s_row[nb + t];
+        rd[t] = (nb + t < cnt) ? r : -1;
+      }
       if (c < n && (c < k0 || right)) {
 #pragma unroll
         for (int q = 0; q < NB; ++q)
-          if (q < nb) S[s_row[q] * ld + c] = u[q];
+          if (ru[q] >= 0) S[ru[q] * ld + c] = u[q];
 #pragma unroll
         for (int t = 0; t < NB; ++t)
-          if (nb + t < cnt) S[s_row[nb + t] * ld + c] = d[t];
+          if (rd[t] >= 0) S[rd[t] * ld + c] = d[t];
       }
     }
     __syncthreads();
