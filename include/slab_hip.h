@@ -130,14 +130,15 @@ int slab_btri_factor_twisted(int dtype, int b, int m, int bs,
                              int* ipiv, int64_t p_bstride, int64_t p_istride, int* info, void* stream);
 
 /* Sweeps with the factors of slab_btri_factor_twisted, in place on f (as
- * slab_btri_solve, non-transposed).  Replaces lusolve_sweep
- * (src_solver/block_tridiag_solver_dense.py:179-202) for twisted factors. */
+ * slab_btri_solve); transpose solves A^T x = f.  Replaces lusolve_sweep and
+ * lusolve_sweep_transpose (src_solver/block_tridiag_solver_dense.py:179-222)
+ * for twisted factors. */
 int slab_btri_solve_twisted(int dtype, int b, int m, int bs,
                             const void* diag, int64_t d_bstride, int64_t d_istride,
                             const void* sub, int64_t l_bstride, int64_t l_istride,
                             const void* sup, int64_t u_bstride, int64_t u_istride,
                             const int* ipiv, int64_t p_bstride, int64_t p_istride,
-                            void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* stream);
+                            void* f, int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream);
 
 /* Inverse triangular factors of every diagonal block after
  * slab_btri_factor (or _twisted): diag's packed LU is replaced by U^{-1}
@@ -153,13 +154,14 @@ int slab_btri_invert_diag(int dtype, int b, int m, int bs, void* diag, int64_t d
                           void* stream);
 
 /* slab_btri_solve_twisted with the factors of slab_btri_invert_diag
- * (lusolve_sweep, src_solver/block_tridiag_solver_dense.py:179-202, with
- * the permutation folded into L^{-1} P). */
+ * (lusolve_sweep / lusolve_sweep_transpose,
+ * src_solver/block_tridiag_solver_dense.py:179-222, with the permutation
+ * folded into L^{-1} P). */
 int slab_btri_solve_twisted_inv(int dtype, int b, int m, int bs,
                                 const void* diag, int64_t d_bstride, int64_t d_istride, const void* linv,
                                 const void* sub, int64_t l_bstride, int64_t l_istride,
                                 const void* sup, int64_t u_bstride, int64_t u_istride,
-                                void* f, int64_t f_bstride, int64_t ldf, int nrhs, void* stream);
+                                void* f, int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream);
 
 /* 1 when blocks of size bs run in the fused one-workgroup-per-chain LU-form
  * kernels (the diagonal block, a right-hand-side chunk and staging fit in

@@ -181,6 +181,32 @@ def bfield_constant(xx, kh):
     return -(kh ** 2) * np.ones(xx.shape[0])
 
 
+def lshaped_waveguide_bumps(xx, amp=1.0, width=0.005, spacing=0.04, removed_row=None, removed_col=None):
+    """Brute-force sum of the benchmark's bump lattice (a field of the
+    benchmark workloads, not of the reference): amp * exp(-|x - x_k|^2 /
+    width^2) over every lattice point x_k = spacing * ((i, j) + 1/2), i, j <
+    1/spacing, except the points (i, rr) with i >= cc and (cc, j) with j >= rr
+    (rr = cc = (1/spacing) // 2 by default), which leave an L-shaped channel.
+    Sums over all lattice points with no neighbourhood cut-off."""
+    xx = np.asarray(xx, dtype=np.float64)
+    n = int(round(1.0 / spacing))
+    rr = n // 2 if removed_row is None else removed_row
+    cc = n // 2 if removed_col is None else removed_col
+    b = np.zeros(xx.shape[0])
+    for i in range(n):
+        for j in range(n):
+            if (j == rr and i >= cc) or (i == cc and j >= rr):
+                continue
+            cx, cy = (i + 0.5) * spacing, (j + 0.5) * spacing
+            b += amp * np.exp(-((xx[:, 0] - cx) ** 2 + (xx[:, 1] - cy) ** 2) / width ** 2)
+    return b
+
+
+def bfield_lshaped_waveguide(xx, kh):
+    """c(x) = -kh^2 (1 - bumps(x)), the coefficient of the waveguide workload."""
+    return -(kh ** 2) * (1.0 - lshaped_waveguide_bumps(xx))
+
+
 def kh_from_ppw(n, ppw):
     """argparse_driver.py:86-90."""
     nwaves = int(n / ppw)
@@ -303,7 +329,11 @@ class HPSOracle:
             acc(pdo.c1, self.Ds["D1"])
         if pdo.c2 is not None:
             acc(pdo.c2, self.Ds["D2"])
-        if pdo.c is not None:
+        c_nodes = getattr(self, "c_nodes", None)
+        if c_nodes is not None:
+            # per-node values of c (nboxes, p^2) in place of the callable
+            A[...] += np.asarray(c_nodes)[boxes][:, :, None] * np.eye(p ** 2)[None]
+        elif pdo.c is not None:
             acc(pdo.c, np.eye(p ** 2))
         return A
 

@@ -53,9 +53,9 @@ hipError_t btri_solve(int b, int m, int bs, const void* diag, int64_t dbs, int64
   auto F = [&](int i) { return View<T>{reinterpret_cast<T*>(f) + (int64_t)i * bs * ldf, ldf, 1, fbs}; };
   auto P = [&](int i) { return ipiv + (int64_t)i * pis; };
   hipError_t e;
-  if (!trans && btri_lu_fused_ok<T>(bs))
+  if (btri_lu_fused_ok<T>(bs))
     return btri_lu_solve_fused<T>(b, m, bs, (T*)diag, dbs, dis, (T*)sub, lbs, lis, (T*)sup, ubs, uis, ipiv, pbs,
-                                  pis, (T*)f, fbs, ldf, nrhs, s);
+                                  pis, (T*)f, fbs, ldf, nrhs, s, trans);
   if (!trans) {
     for (int i = 0; i < m; ++i) {
       if (i > 0 && (e = gemm<T>(b, bs, nrhs, bs, T(-1), L(i - 1), F(i - 1), T(1), F(i), s)) != hipSuccess) return e;
@@ -244,16 +244,18 @@ int slab_btri_solve_twisted(int dtype, int b, int m, int bs, const void* diag, i
                             int64_t d_istride, const void* sub, int64_t l_bstride, int64_t l_istride,
                             const void* sup, int64_t u_bstride, int64_t u_istride, const int* ipiv,
                             int64_t p_bstride, int64_t p_istride, void* f, int64_t f_bstride, int64_t ldf, int nrhs,
-                            void* stream) {
+                            int transpose, void* stream) {
   if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs || slab_btri_fused(dtype, bs) != 1) return -1;
   if (nrhs == 0 || b == 0) return 0;
   if (dtype == SLAB_F64)
     return status(btri_lu_solve_twisted<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub,
                                                 l_bstride, l_istride, (double*)sup, u_bstride, u_istride, ipiv,
-                                                p_bstride, p_istride, (double*)f, f_bstride, ldf, nrhs, S(stream)));
+                                                p_bstride, p_istride, (double*)f, f_bstride, ldf, nrhs, S(stream),
+                                                nullptr, transpose != 0));
   return status(btri_lu_solve_twisted<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
                                              l_istride, (float*)sup, u_bstride, u_istride, ipiv, p_bstride,
-                                             p_istride, (float*)f, f_bstride, ldf, nrhs, S(stream)));
+                                             p_istride, (float*)f, f_bstride, ldf, nrhs, S(stream), nullptr,
+                                             transpose != 0));
 }
 
 int slab_btri_invert_diag(int dtype, int b, int m, int bs, void* diag, int64_t d_bstride, int64_t d_istride,
@@ -279,17 +281,18 @@ int slab_btri_invert_diag(int dtype, int b, int m, int bs, void* diag, int64_t d
 int slab_btri_solve_twisted_inv(int dtype, int b, int m, int bs, const void* diag, int64_t d_bstride,
                                 int64_t d_istride, const void* linv, const void* sub, int64_t l_bstride,
                                 int64_t l_istride, const void* sup, int64_t u_bstride, int64_t u_istride, void* f,
-                                int64_t f_bstride, int64_t ldf, int nrhs, void* stream) {
+                                int64_t f_bstride, int64_t ldf, int nrhs, int transpose, void* stream) {
   if (b < 0 || m < 1 || bs < 1 || nrhs < 0 || ldf < nrhs || linv == nullptr || slab_btri_fused(dtype, bs) != 1)
     return -1;
   if (nrhs == 0 || b == 0) return 0;
   if (dtype == SLAB_F64)
     return status(btri_lu_solve_twisted<double>(b, m, bs, (double*)diag, d_bstride, d_istride, (double*)sub,
                                                 l_bstride, l_istride, (double*)sup, u_bstride, u_istride, nullptr, 0,
-                                                0, (double*)f, f_bstride, ldf, nrhs, S(stream), (const double*)linv));
+                                                0, (double*)f, f_bstride, ldf, nrhs, S(stream), (const double*)linv,
+                                                transpose != 0));
   return status(btri_lu_solve_twisted<float>(b, m, bs, (float*)diag, d_bstride, d_istride, (float*)sub, l_bstride,
                                              l_istride, (float*)sup, u_bstride, u_istride, nullptr, 0, 0, (float*)f,
-                                             f_bstride, ldf, nrhs, S(stream), (const float*)linv));
+                                             f_bstride, ldf, nrhs, S(stream), (const float*)linv, transpose != 0));
 }
 
 int slab_btri_inv_fused(int dtype, int bs) {

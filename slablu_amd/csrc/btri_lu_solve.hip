@@ -19,7 +19,17 @@ namespace {
 // is loaded at hand-over), its right-hand-side rows and its pivots are
 // already being loaded into registers, so global latency is off the
 // critical path.  Workgroups past c0's chains take c1's chains.
-template <typename T, int NR, int PF, bool INV>
+//
+// TRANS solves the transposed system (lusolve_sweep_transpose,
+// block_tridiag_solver_dense.py:205-222).  With the factorization
+// T = LL UU (LL: diagonal P_i^T L_i, sub-diagonal Lfac_i; UU: diagonal U_i,
+// super-diagonal Ufac_i), T^T = UU^T LL^T, so the same stage sequence runs
+// with every block staged transposed: DIAGL(i) solves with U_i^T (lower,
+// non-unit), SUB(i) subtracts Ufac_i^T, DIAGU(i) solves with L_i^T (upper,
+// unit) and then applies P_i^T to its result, SUP(i) subtracts Lfac_i^T.
+// With inverse factors DIAGL multiplies by (U^{-1})^T and DIAGU by
+// (L^{-1} P)^T.
+template <typename T, int NR, int PF, bool INV, bool TRANS>
 __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solve(int bs, ChainSet<T> c0, ChainSet<T> c1, int64_t ldf,
                                                        int nrhs, int mode) {
   // rows per thread in the block products, for the largest block the fused
@@ -69,21 +79,32 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
       return 2;
     };
     // inverse factors: DIAGL multiplies by L^{-1} P (linv), DIAGU by U^{-1}
-    // (diag), both plain block products
+    // (diag), both plain block products (transposed: DIAGL by the
+    // transposed U^{-1}, DIAGU by the transposed L^{-1} P)
     // (compile-time, so each variant keeps only the registers its stages need)
     constexpr bool inv = INV;
     // the last block's DIAGU of a full sweep reuses the resident DIAGL block
     auto resident = [&](int k, int i) -> bool { return !inv && mode == 0 && k == 2 && i == m - 1; };
+    // the stage whose block needs the pivots: the permutation enters before
+    // the unit-lower solve, or (transposed) after the unit-upper one
+    constexpr int kperm = TRANS ? 2 : 0;
     T* fb = a.f + b * a.fbs;
     const int* pb = a.ipiv + b * a.pbs;
     T* Xc = B0;
     T* Y = B1;
     auto frow = [&](int i, int r) -> T* { return fb + (int64_t)i * a.fis + (int64_t)r * ldf + col0; };
     auto block_src = [&](int k, int i) -> const T* {
-      if (k == 1) return a.sub + b * a.lbs + (int64_t)i * a.lis;
-      if (k == 3) return a.sup + b * a.ubs + (int64_t)i * a.uis;
-      if (k == 0 && inv) return a.linv + b * a.dbs + (int64_t)i * a.dis;
+      // transposed sweeps use Ufac in the forward and Lfac in the backward half
+      if (k == (TRANS ? 3 : 1)) return a.sub + b * a.lbs + (int64_t)i * a.lis;
+      if (k == (TRANS ? 1 : 3)) return a.sup + b * a.ubs + (int64_t)i * a.uis;
+      if (k == (TRANS ? 2 : 0) && inv) return a.linv + b * a.dbs + (int64_t)i * a.dis;
       return a.diag + b * a.dbs + (int64_t)i * a.dis;
+    };
+    // LDS slot of element e = r * bs + c of a block: row-major, or
+    // transposed for the transposed sweeps
+    auto slot = [&](int e) -> int {
+      const int r = idiv(e, rbs), c = e - r * bs;
+      return TRANS ? c * ld + r : r * ld + c;
     };
     // forward sweeps start at the first block whose right-hand side tile
     // has a nonzero: the blocks above it solve to zero and already hold
@@ -129,7 +150,7 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
           pfx[u] = *(frow(fi, r < bs ? r : bs - 1) + (c < cm1 ? c : cm1));
         }
       }
-      if (k == 0 && !inv) {
+      if (k == kperm && !inv) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int r = tid + 256 * u;
@@ -145,7 +166,7 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int e = tid + 256 * u;
-          if (e < nn) Mt[e + idiv(e, rbs)] = pf[u];
+          if (e < nn) Mt[slot(e)] = pf[u];
         }
         const T* src = block_src(k, i);
         // the rest of the block, eight loads in flight per thread
@@ -159,7 +180,7 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int e = e0 + 256 * u;
-            if (e < nn) Mt[e + idiv(e, rbs)] = v[u];
+            if (e < nn) Mt[slot(e)] = v[u];
           }
         }
       }
@@ -170,7 +191,7 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
           if (e < nf) Xc[r * ldx + c] = (c < ncols) ? pfx[u] : T(0);
         }
       }
-      if (k == 0 && !inv) {
+      if (k == kperm && !inv) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int r = tid + 256 * u;
@@ -211,6 +232,15 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
       if (k == 0) {
         if (inv) {
           block_set(Xc, Y);
+        } else if constexpr (TRANS) {
+          // U_i^T y = x: the lower triangle (with the diagonal) of the
+          // transposed LU block
+          for (int e = tid; e < nf; e += 256) {
+            const int r = e >> lgnr, c = e & (nr - 1);
+            Y[r * ldx + c] = Xc[r * ldx + c];
+          }
+          __syncthreads();
+          tri_solve_n<T, false, false, NR>(Mat, bs, Y, ldx);
         } else {
           piv_to_perm(P, bs, Q);
           for (int e = tid; e < nf; e += 256) {
@@ -231,13 +261,34 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
       } else {
         const bool own = mode == 0 && i == m - 1;
         T* Z = own ? Y : Xc;
-        if (inv) block_set(Z, Z);
-        else tri_solve_n<T, true, false, NR>(Mat, bs, Z, ldx);
-        store_f(i, Z);
-        if (!own) {
-          T* t = Xc;
-          Xc = Y;
-          Y = t;
+        if (!inv && TRANS) {
+          // L_i^T w = z (the strict upper triangle of the transposed LU
+          // block, unit diagonal), then x = P_i^T w: row r of w lands in
+          // row Q[r] of x, in the buffer that is not Z
+          tri_solve_n<T, true, true, NR>(Mat, bs, Z, ldx);
+          T* O = own ? Xc : Y;
+          piv_to_perm(P, bs, Q);
+          for (int e = tid; e < nf; e += 256) {
+            const int r = e >> lgnr, c = e & (nr - 1);
+            O[Q[r] * ldx + c] = Z[r * ldx + c];
+          }
+          __syncthreads();
+          store_f(i, O);
+          // the result continues in Y
+          if (own) {
+            T* t = Xc;
+            Xc = Y;
+            Y = t;
+          }
+        } else {
+          if (inv) block_set(Z, Z);
+          else tri_solve_n<T, true, false, NR>(Mat, bs, Z, ldx);
+          store_f(i, Z);
+          if (!own) {
+            T* t = Xc;
+            Xc = Y;
+            Y = t;
+          }
         }
       }
       __syncthreads();
@@ -247,23 +298,34 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
   }
 }
 
-template <typename T, int NR, int PF>
-hipError_t launch_lu_solve(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
-                           hipStream_t s) {
+template <typename T, int NR, int PF, bool INV, bool TRANS>
+hipError_t launch_lu_solve_k(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
+                             hipStream_t s) {
   const size_t lds = lu_solve_lds<T>(bs, NR);
-  const bool inv = c0.linv != nullptr;
-  const void* fn = inv ? reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF, true>)
-                       : reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF, false>);
+  const void* fn = reinterpret_cast<const void*>(&k_btri_lu_solve<T, NR, PF, INV, TRANS>);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const int nt = c0.nchains + c1.nchains;
   if (nt <= 0) return hipSuccess;
   dim3 grid(nt < 65535 ? nt : 65535, (nrhs + NR - 1) / NR);
-  if (inv)
-    hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF, true>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
-  else
-    hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF, false>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
+  hipLaunchKernelGGL((k_btri_lu_solve<T, NR, PF, INV, TRANS>), grid, dim3(256), lds, s, bs, c0, c1, ldf, nrhs, mode);
   return hipGetLastError();
+}
+
+template <typename T, int NR, int PF>
+hipError_t launch_lu_solve(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
+                           bool trans, hipStream_t s) {
+  const bool inv = c0.linv != nullptr;
+  if (inv) {
+    // 32-column inverse-factor sweeps always run with the small prefetch
+    // (launch_lu_solve_pf); the large one would spill, so it is not built
+    if constexpr (NR == 32 && PF != 8) return hipErrorInvalidValue;
+    else
+      return trans ? launch_lu_solve_k<T, NR, PF, true, true>(bs, c0, c1, ldf, nrhs, mode, s)
+                   : launch_lu_solve_k<T, NR, PF, true, false>(bs, c0, c1, ldf, nrhs, mode, s);
+  }
+  return trans ? launch_lu_solve_k<T, NR, PF, false, true>(bs, c0, c1, ldf, nrhs, mode, s)
+               : launch_lu_solve_k<T, NR, PF, false, false>(bs, c0, c1, ldf, nrhs, mode, s);
 }
 
 // PF: block elements prefetched per thread (the rest of a block is loaded
@@ -271,14 +333,14 @@ hipError_t launch_lu_solve(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1,
 // the register file
 template <typename T, int NR>
 hipError_t launch_lu_solve_pf(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
-                              hipStream_t s) {
+                              bool trans, hipStream_t s) {
   constexpr int PFBIG = sizeof(T) == 8 ? (NR >= 32 ? 8 : (NR >= 16 ? 16 : 40)) : 40;
   // inverse factors with 32 columns: a small prefetch keeps two workgroups
   // per CU (their LDS fits twice), which hides more than a whole-block
   // prefetch in one
   if ((int64_t)bs * bs <= 256 * 8 || (NR == 32 && c0.linv != nullptr))
-    return launch_lu_solve<T, NR, 8>(bs, c0, c1, ldf, nrhs, mode, s);
-  return launch_lu_solve<T, NR, PFBIG>(bs, c0, c1, ldf, nrhs, mode, s);
+    return launch_lu_solve<T, NR, 8>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+  return launch_lu_solve<T, NR, PFBIG>(bs, c0, c1, ldf, nrhs, mode, trans, s);
 }
 
 // Single-column sweeps with inverse factors (FP32, bs <= 104, bs % 4 == 0):
@@ -403,7 +465,7 @@ __global__ __launch_bounds__(256) void k_btri_inv1(int bs, ChainSet<float> c0, C
 
 template <typename T>
 hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int64_t ldf, int nrhs, int mode,
-                      hipStream_t s) {
+                      bool trans, hipStream_t s) {
   if constexpr (sizeof(T) == 4) {
     // 16-byte vectors: every block offset and base a multiple of 4 floats
     auto al = [](const ChainSet<T>& c) {
@@ -412,7 +474,7 @@ hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int6
       return c.linv != nullptr && p4(c.linv) && p4(c.diag) && p4(c.sub) && p4(c.sup) && c.dbs % 4 == 0 &&
              c.dis % 4 == 0 && c.lbs % 4 == 0 && c.lis % 4 == 0 && c.ubs % 4 == 0 && c.uis % 4 == 0;
     };
-    if (nrhs == 1 && c0.linv != nullptr && al(c0) && al(c1) && bs <= kInv1MaxBs && bs % 4 == 0) {
+    if (!trans && nrhs == 1 && c0.linv != nullptr && al(c0) && al(c1) && bs <= kInv1MaxBs && bs % 4 == 0) {
       const int nt = c0.nchains + c1.nchains;
       if (nt <= 0) return hipSuccess;
       hipLaunchKernelGGL(k_btri_inv1, dim3(nt < 65535 ? nt : 65535), dim3(256), 0, s, bs,
@@ -428,12 +490,12 @@ hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int6
   while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
   if (lu_solve_lds<T>(bs, nr) > kLdsMax || bs > (sizeof(T) == 8 ? 112 : 160)) return hipErrorInvalidValue;
   switch (nr) {
-    case 1: return launch_lu_solve_pf<T, 1>(bs, c0, c1, ldf, nrhs, mode, s);
-    case 2: return launch_lu_solve_pf<T, 2>(bs, c0, c1, ldf, nrhs, mode, s);
-    case 4: return launch_lu_solve_pf<T, 4>(bs, c0, c1, ldf, nrhs, mode, s);
-    case 8: return launch_lu_solve_pf<T, 8>(bs, c0, c1, ldf, nrhs, mode, s);
-    case 16: return launch_lu_solve_pf<T, 16>(bs, c0, c1, ldf, nrhs, mode, s);
-    default: return launch_lu_solve_pf<T, 32>(bs, c0, c1, ldf, nrhs, mode, s);
+    case 1: return launch_lu_solve_pf<T, 1>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+    case 2: return launch_lu_solve_pf<T, 2>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+    case 4: return launch_lu_solve_pf<T, 4>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+    case 8: return launch_lu_solve_pf<T, 8>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+    case 16: return launch_lu_solve_pf<T, 16>(bs, c0, c1, ldf, nrhs, mode, trans, s);
+    default: return launch_lu_solve_pf<T, 32>(bs, c0, c1, ldf, nrhs, mode, trans, s);
   }
 }
 
@@ -442,21 +504,24 @@ hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int6
 template <typename T>
 hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
-                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s) {
+                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s, bool trans) {
   ChainSet<T> c0{diag, dbs, dis, sub, lbs, lis, sup, ubs, uis, const_cast<int*>(ipiv), pbs, pis, nullptr,
                  f, fbs, (int64_t)bs * ldf, m, nb, 0};
   ChainSet<T> none{};
-  return solve_sets<T>(bs, c0, none, ldf, nrhs, 0, s);
+  return solve_sets<T>(bs, c0, none, ldf, nrhs, 0, trans, s);
 }
 
 // Sweeps with the factors of btri_lu_factor_twisted: forward down both
 // chains, the middle block's right-hand side less both chains' couplings,
-// the middle block, then backward out along both chains.
+// the middle block, then backward out along both chains.  The top chain's
+// last Lfac/Ufac (towards the middle block) sit in sub_{k-1}/sup_{k-1}, the
+// bottom chain's in sup_k/sub_k; the transposed sweep couples the middle
+// block through the Ufac of both chains (transposed) on the way in.
 template <typename T>
 hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                  int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
                                  int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s,
-                                 const T* linv) {
+                                 const T* linv, bool trans) {
   const int k = m / 2, lb = m - 1 - k;
   int* iv = const_cast<int*>(ipiv);
   const int64_t fis = (int64_t)bs * ldf;
@@ -466,33 +531,36 @@ hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, in
                                 f, fbs, ldf, linv);
   hipError_t e;
   if (k > 0) {
-    if ((e = solve_sets<T>(bs, top, bot, ldf, nrhs, 1, s)) != hipSuccess) return e;
+    if ((e = solve_sets<T>(bs, top, bot, ldf, nrhs, 1, trans, s)) != hipSuccess) return e;
     View<T> Fk{f + (int64_t)k * fis, ldf, 1, fbs};
-    View<T> Lt{sub + (int64_t)(k - 1) * lis, bs, 1, lbs};
-    if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Lt, View<T>{f + (int64_t)(k - 1) * fis, ldf, 1, fbs}, T(1), Fk, s)) !=
+    // coupling of the top chain's last block into the middle block:
+    // Lfac = sub_{k-1}, or Ufac^T = sup_{k-1}^T when transposed
+    View<T> Ct = trans ? View<T>{sup + (int64_t)(k - 1) * uis, 1, bs, ubs} : View<T>{sub + (int64_t)(k - 1) * lis, bs, 1, lbs};
+    if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Ct, View<T>{f + (int64_t)(k - 1) * fis, ldf, 1, fbs}, T(1), Fk, s)) !=
         hipSuccess)
       return e;
     if (lb > 0) {
-      View<T> Lb{sup + (int64_t)k * uis, bs, 1, ubs};
-      if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Lb, View<T>{f + (int64_t)(k + 1) * fis, ldf, 1, fbs}, T(1), Fk,
+      // bottom chain: Lfac' = sup_k, or Ufac'^T = sub_k^T when transposed
+      View<T> Cb = trans ? View<T>{sub + (int64_t)k * lis, 1, bs, lbs} : View<T>{sup + (int64_t)k * uis, bs, 1, ubs};
+      if ((e = gemm<T>(nb, bs, nrhs, bs, T(-1), Cb, View<T>{f + (int64_t)(k + 1) * fis, ldf, 1, fbs}, T(1), Fk,
                        s)) != hipSuccess)
         return e;
     }
   }
   ChainSet<T> mid{diag + (int64_t)k * dis, dbs, dis, sub, lbs, lis, sup, ubs, uis, iv + (int64_t)k * pis, pbs, pis,
                   nullptr, f + (int64_t)k * fis, fbs, fis, 1, nb, 0, linv != nullptr ? linv + (int64_t)k * dis : nullptr};
-  if ((e = solve_sets<T>(bs, mid, none, ldf, nrhs, 0, s)) != hipSuccess) return e;
-  if (k > 0) return solve_sets<T>(bs, top, bot, ldf, nrhs, 2, s);
+  if ((e = solve_sets<T>(bs, mid, none, ldf, nrhs, 0, trans, s)) != hipSuccess) return e;
+  if (k > 0) return solve_sets<T>(bs, top, bot, ldf, nrhs, 2, trans, s);
   return hipSuccess;
 }
 
 #define SLAB_BTRI_LU_SOLVE_INST(T)                                                                                \
   template hipError_t btri_lu_solve_fused<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,       \
                                              int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t, int64_t, \
-                                             int, hipStream_t);                                                    \
+                                             int, hipStream_t, bool);                                              \
   template hipError_t btri_lu_solve_twisted<T>(int, int, int, T*, int64_t, int64_t, T*, int64_t, int64_t, T*,     \
                                                int64_t, int64_t, const int*, int64_t, int64_t, T*, int64_t,        \
-                                               int64_t, int, hipStream_t, const T*);
+                                               int64_t, int, hipStream_t, const T*, bool);
 SLAB_BTRI_LU_SOLVE_INST(double)
 SLAB_BTRI_LU_SOLVE_INST(float)
 

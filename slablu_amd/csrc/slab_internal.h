@@ -106,7 +106,8 @@ hipError_t btri_lu_factor_fused(int nb, int m, int bs, T* diag, int64_t dbs, int
 template <typename T>
 hipError_t btri_lu_solve_fused(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
-                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s);
+                               int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s,
+                               bool trans = false);
 // twisted variants: blocks 0..m/2-1 top-down, m-1..m/2+1 bottom-up, then the
 // middle block m/2 (same storage; the factors serve the twisted sweeps only)
 template <typename T>
@@ -114,12 +115,13 @@ hipError_t btri_lu_factor_twisted(int nb, int m, int bs, T* diag, int64_t dbs, i
                                   int64_t lis, T* sup, int64_t ubs, int64_t uis, int* ipiv, int64_t pbs,
                                   int64_t pis, int* info, hipStream_t s);
 // linv (optional): L^{-1} P of every diagonal block in diag's layout, diag
-// then holding U^{-1} (getri_factors): every sweep stage is a block product
+// then holding U^{-1} (getri_factors): every sweep stage is a block product.
+// trans: solve with the transposed matrix.
 template <typename T>
 hipError_t btri_lu_solve_twisted(int nb, int m, int bs, T* diag, int64_t dbs, int64_t dis, T* sub, int64_t lbs,
                                  int64_t lis, T* sup, int64_t ubs, int64_t uis, const int* ipiv, int64_t pbs,
                                  int64_t pis, T* f, int64_t fbs, int64_t ldf, int nrhs, hipStream_t s,
-                                 const T* linv = nullptr);
+                                 const T* linv = nullptr, bool trans = false);
 // block-Thomas factorization / sweeps with explicit diagonal inverses
 template <typename T>
 bool btri_inv_fused_ok(int bs);

@@ -46,12 +46,12 @@ def _load():
         "slab_btri_inv_fused": ([_int, _int], _int),
         "slab_btri_invert_diag": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp], _int),
         "slab_btri_solve_twisted_inv": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64,
-                                         _i64, _vp, _i64, _i64, _int, _vp], _int),
+                                         _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
         "slab_btri_fused": ([_int, _int], _int),
         "slab_btri_factor_twisted": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                                       _i64, _i64, _vp, _vp], _int),
         "slab_btri_solve_twisted": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
-                                     _i64, _i64, _vp, _i64, _i64, _int, _vp], _int),
+                                     _i64, _i64, _vp, _i64, _i64, _int, _int, _vp], _int),
         "slab_btri_inv_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                                   _vp, _vp, _vp], _int),
         "slab_btri_inv_solve": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
@@ -262,19 +262,18 @@ def btri_solve(diag, sub, sup, ipiv, f, transpose=False, twisted=False, linv=Non
     b, m, bs, _ = diag.shape
     _require(f.dim() == 3 and f.shape[0] == b and f.shape[1] == m * bs and f.stride(2) == 1, "btri rhs layout")
     if linv is not None:
-        _require(twisted and not transpose, "inverse-factor sweeps need twisted factors")
+        _require(twisted, "inverse-factor sweeps need twisted factors")
         _check(LIB.slab_btri_solve_twisted_inv(_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1),
                                                _p(linv), _p(sub), sub.stride(0), sub.stride(1) if m > 1 else 0,
                                                _p(sup), sup.stride(0), sup.stride(1) if m > 1 else 0, _p(f),
-                                               f.stride(0), f.stride(1), f.shape[2], _stream()),
+                                               f.stride(0), f.stride(1), f.shape[2], int(transpose), _stream()),
                "slab_btri_solve_twisted_inv")
         return f
     args = (_dt(diag), b, m, bs, _p(diag), diag.stride(0), diag.stride(1), _p(sub), sub.stride(0),
             sub.stride(1) if m > 1 else 0, _p(sup), sup.stride(0), sup.stride(1) if m > 1 else 0, _p(ipiv),
             ipiv.stride(0), ipiv.stride(1), _p(f), f.stride(0), f.stride(1), f.shape[2])
     if twisted:
-        _require(not transpose, "twisted factors have no transposed sweep")
-        _check(LIB.slab_btri_solve_twisted(*args, _stream()), "slab_btri_solve_twisted")
+        _check(LIB.slab_btri_solve_twisted(*args, int(transpose), _stream()), "slab_btri_solve_twisted")
     else:
         _check(LIB.slab_btri_solve(*args, int(transpose), _stream()), "slab_btri_solve")
     return f

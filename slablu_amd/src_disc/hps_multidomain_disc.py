@@ -176,12 +176,13 @@ class HPS_Multidomain:
         return uu.reshape(-1, uu.shape[-1]), torch.linalg.norm(res).item()
 
     # -- face operator ----------------------------------------------------------
-    def apply_A(self, x_face, DtN=None):
-        """y = A x for a face vector (nface, nrhs), matrix-free from the DtNs."""
+    def apply_A(self, x_face, DtN=None, transpose=False):
+        """y = A x (or A^T x) for a face vector (nface, nrhs), matrix-free
+        from the DtNs: A = sum_b P_b^T DtN_b P_b, A^T = sum_b P_b^T DtN_b^T P_b."""
         DtN = self.leaf["DtN"] if DtN is None else DtN
         xb = x_face.to(DtN.dtype)[self.Iext_d].contiguous()
         zb = torch.empty_like(xb)
-        hip.gemm(DtN, xb, zb)
+        hip.gemm(DtN.transpose(1, 2) if transpose else DtN, xb, zb)
         y = torch.zeros(self.nface, x_face.shape[-1], dtype=DtN.dtype, device=DtN.device)
         y.index_add_(0, self.Iext_d.reshape(-1), zb.reshape(-1, x_face.shape[-1]))
         return y

@@ -122,10 +122,8 @@ class Block_Tridiagonal:
                 raise NotImplementedError("transposed sweeps need form='lu'")
             hip.btri_inv_solve(d, l, u, fv)
         else:
-            if transpose and self.twisted:
-                raise NotImplementedError("transposed sweeps need the non-twisted factorization")
             if self.linv is not None:
-                hip.btri_solve(self.uinv[b_start:b_end], l, u, None, fv, twisted=True,
+                hip.btri_solve(self.uinv[b_start:b_end], l, u, None, fv, transpose=transpose, twisted=True,
                                linv=self.linv[b_start:b_end])
             else:
                 hip.btri_solve(d, l, u, self.pivs[b_start:b_end], fv, transpose=transpose, twisted=self.twisted)

@@ -21,6 +21,14 @@ block products.  Each block is kept in inverse form (S^{-1} and
 G = S^{-1} * coupling towards the middle, FP64), so a solve is a sequence of
 matrix products: forward sweeps down both chains, the middle block, then
 back-substitution out to both ends.
+
+In the chain order [top chain, bottom chain, middle] the factorization is
+T = LL UU with LL lower block-bidiagonal (diagonal S_t, sub-diagonal the
+couplings Cl_t, middle row [sub_{k-1}, sup_k, M]) and UU unit upper
+(super-diagonal G_t).  The transposed solve (the interface half of the
+transposed two-level solve, slablu_solver.py:963-1025 for the operator)
+runs T^T = UU^T LL^T: a forward sweep with the G_t^T, then a backward
+sweep with the S_t^{-T} and Cl_t^T.
 """
 import torch
 
@@ -146,6 +154,9 @@ class HBS_BlockTridiagonal:
         Bcol[m - 2] += sup[m - 2]
         self.C0 = sup[m - 1].contiguous()        # block (b-1, 0)
         self.Cl = sub[m - 2].contiguous()        # block (b-1, b-2)
+        # the nonzero blocks of B (for the transposed solve)
+        self.B0, self.Bl = Bcol[0].clone(), Bcol[m - 2].clone()
+        self.Wt = None                           # T^{-T} C^T, formed at the first transposed solve
         W = self.inner.solve(Bcol.reshape((m - 1) * n, n)).reshape(m - 1, n, n)
         S = diag[m - 1].clone().unsqueeze(0)
         hip.gemm(self.C0, W[0].unsqueeze(0), S, alpha=-1.0, beta=1.0)
@@ -159,27 +170,91 @@ class HBS_BlockTridiagonal:
         self.dtype, self.device = dtype, device
         self.build_mode = "lowmem"
 
-    def _solve_cyclic(self, f):
+    def _solve_cyclic(self, f, transpose=False):
+        """[T B; C D] [x; xb] = [f; fb]: y = T^{-1} f, xb = S^{-1}(fb - C y),
+        x = y - W xb.  Transposed: y = T^{-T} f, xb = S^{-T}(fb - B^T y),
+        x = y - T^{-T} C^T xb."""
         m, n = self.b, self.n
         nrhs = f.shape[-1]
+        tr = lambda X: X.transpose(-1, -2)
         fb = f.to(self.device, self.dtype).reshape(m * n, nrhs)
-        y = self.inner.solve(fb[:(m - 1) * n]).reshape(m - 1, n, nrhs)
+        y = self.inner.solve(fb[:(m - 1) * n], transpose=transpose).reshape(m - 1, n, nrhs)
         r = fb[(m - 1) * n:].clone().unsqueeze(0)
-        hip.gemm(self.C0, y[0].unsqueeze(0), r, alpha=-1.0, beta=1.0)
-        hip.gemm(self.Cl, y[m - 2].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+        if transpose:
+            hip.gemm(tr(self.B0), y[0].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+            if m > 2:
+                hip.gemm(tr(self.Bl), y[m - 2].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+            if self.Wt is None:
+                Ct = torch.zeros(m - 1, n, n, dtype=self.dtype, device=self.device)
+                Ct[0] += tr(self.C0)
+                Ct[m - 2] += tr(self.Cl)
+                self.Wt = self.inner.solve(Ct.reshape((m - 1) * n, n), transpose=True).contiguous()
+            Sm, Wm = tr(self.Sinv), self.Wt
+        else:
+            hip.gemm(self.C0, y[0].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+            hip.gemm(self.Cl, y[m - 2].unsqueeze(0), r, alpha=-1.0, beta=1.0)
+            Sm, Wm = self.Sinv, self.W
         xb = torch.empty(1, n, nrhs, dtype=self.dtype, device=self.device)
-        hip.gemm(self.Sinv, r, xb)
+        hip.gemm(Sm, r, xb)
         x = torch.empty(m * n, nrhs, dtype=self.dtype, device=self.device)
         xs = y.reshape((m - 1) * n, nrhs).clone().unsqueeze(0)
-        hip.gemm(self.W.unsqueeze(0), xb, xs, alpha=-1.0, beta=1.0)
+        hip.gemm(Wm.unsqueeze(0), xb, xs, alpha=-1.0, beta=1.0)
         x[:(m - 1) * n] = xs[0]
         x[(m - 1) * n:] = xb[0]
         return x
 
-    def solve(self, f):
-        """f (b*n, nrhs) -> solution (b*n, nrhs); f is not modified."""
+    def _solve_transposed(self, f):
+        """T^T x = f with the factors of the twisted elimination: forward
+        z_t = f_t - G_{t-1}^T z_{t-1} down both chains, z_k = f_k - G^T z of
+        both chain ends, x_k = M^{-T} z_k, then backward x_t = S_t^{-T}(z_t -
+        Cl_{t+1}^T x_{t+1}) out to both ends (the middle couplings sub_{k-1},
+        sup_k in place of Cl_L)."""
+        m, n, k, L = self.b, self.n, self.k, self.L
+        nrhs = f.shape[-1]
+        tr = lambda X: X.transpose(-1, -2)
+        fb = f.to(self.device, self.dtype).reshape(m, n, nrhs)
+        x = torch.empty(m, n, nrhs, dtype=self.dtype, device=self.device)
+        z_mid = fb[k].clone().unsqueeze(0)
+        if L > 0:
+            pad = self.pad
+            bot = [m - 1 - (t - pad) if t >= pad else None for t in range(L)]
+            Z = torch.zeros(2, L, n, nrhs, dtype=self.dtype, device=self.device)
+            for t in range(L):
+                Z[0, t] = fb[t]
+                if bot[t] is not None:
+                    Z[1, t] = fb[bot[t]]
+                if t > 0:
+                    hip.gemm(tr(self.G[:, t - 1]), Z[:, t - 1], Z[:, t], alpha=-1.0, beta=1.0)
+            hip.gemm(tr(self.G[0, L - 1]).unsqueeze(0), Z[0, L - 1].unsqueeze(0), z_mid, alpha=-1.0, beta=1.0)
+            if self.has_bot:
+                hip.gemm(tr(self.G[1, L - 1]).unsqueeze(0), Z[1, L - 1].unsqueeze(0), z_mid, alpha=-1.0, beta=1.0)
+        xk = torch.empty(1, n, nrhs, dtype=self.dtype, device=self.device)
+        hip.gemm(tr(self.Minv), z_mid, xk)
+        x[k] = xk[0]
+        if L > 0:
+            X = torch.empty(2, L, n, nrhs, dtype=self.dtype, device=self.device)
+            W = torch.empty(2, n, nrhs, dtype=self.dtype, device=self.device)
+            for t in range(L - 1, -1, -1):
+                W.copy_(Z[:, t])
+                if t == L - 1:
+                    hip.gemm(tr(self.sub_mid).unsqueeze(0), xk, W[0:1], alpha=-1.0, beta=1.0)
+                    if self.has_bot:
+                        hip.gemm(tr(self.sup_mid).unsqueeze(0), xk, W[1:2], alpha=-1.0, beta=1.0)
+                else:
+                    hip.gemm(tr(self.Cl[:, t + 1]), X[:, t + 1], W, alpha=-1.0, beta=1.0)
+                hip.gemm(tr(self.Sinv[:, t]), W, X[:, t])
+            x[:L] = X[0]
+            for t in range(pad, L):
+                x[m - 1 - (t - pad)] = X[1, t]
+        return x.reshape(m * n, nrhs)
+
+    def solve(self, f, transpose=False):
+        """f (b*n, nrhs) -> solution (b*n, nrhs) of T x = f, or T^T x = f
+        with transpose; f is not modified."""
         if self.cyclic:
-            return self._solve_cyclic(f)
+            return self._solve_cyclic(f, transpose)
+        if transpose:
+            return self._solve_transposed(f)
         m, n, k, L = self.b, self.n, self.k, self.L
         nrhs = f.shape[-1]
         fb = f.to(self.device, self.dtype).reshape(m, n, nrhs)

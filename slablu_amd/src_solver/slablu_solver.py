@@ -34,6 +34,7 @@ preconditioned by the FP32 two-level solve); refine_method='richardson' runs
 plain iterative refinement instead (one FP32 solve per correction).
 """
 import time
+import warnings
 from functools import reduce
 
 import numpy as np
@@ -72,11 +73,14 @@ def _blockdiag(blocks):
 
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
-                 device=None, refine_tol=1e-8, refine_maxit=12, refine_method="fgmres", refine_restart=10):
+                 device=None, refine_tol=1e-8, refine_maxit=12, refine_method="fgmres", refine_restart=10,
+                 check_pivots="warn", on_unconverged="warn"):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if check_pivots not in ("warn", "raise", "ignore") or on_unconverged not in ("warn", "raise", "ignore"):
+            raise ValueError("check_pivots / on_unconverged must be 'warn', 'raise' or 'ignore'")
         self.kh = kh
         self.periodic_bc = periodic_bc
         self.buf_constant = buf_constant
@@ -85,6 +89,15 @@ class Domain_Solver:
             raise ValueError("refine_method must be 'fgmres' or 'richardson'")
         self.refine_tol, self.refine_maxit = refine_tol, refine_maxit
         self.refine_method, self.refine_restart = refine_method, refine_restart
+        # exact zero pivots of the leaf, slab and interface factorizations
+        # (LAPACK's info) are read back at the end of build(); the reference
+        # leaves them unchecked (torch lu_factor returns inf/nan silently)
+        self.check_pivots = check_pivots
+        # a refinement that stops at refine_maxit without meeting refine_tol
+        self.on_unconverged = on_unconverged
+        self.refine_converged = None
+        self.refine_history, self.refine_residuals = [], []
+        self.solve_info = {}
         # stage timings of build() from events on the stream (no host
         # synchronization between stages)
         self.stage_sync = True
@@ -209,13 +222,33 @@ class Domain_Solver:
             t.update(toc_DtN=float("nan"), toc_Aloc=float("nan"), toc_Ared=float("nan"), toc_Tred=float("nan"),
                      toc_Tform=float("nan"))
         mem = (self.hps.leaf["M"].numel() * self.hps.leaf["M"].element_size() + self.Aloc_solver.nbytes +
-               self.Ared_solver.nbytes + self.Tred_solver.nbytes) / 1e9
+               self.Ared_solver.nbytes + (self.Tred_solver.nbytes if self.Tred_solver is not None else 0)) / 1e9
         info = dict(toc_build=toc_build, mem_build=mem, toc_assembly=t["toc_DtN"], **t)
+        info["zero_pivots"] = self.zero_pivots()
         if verbose:
             print("SLABLU BUILD (%s): DtN %.4f s, Aloc %.4f s, Ared %.4f s, Tred %.4f s, total %.4f s, %.2f GB"
                   % (self.precision, t["toc_DtN"], t["toc_Aloc"], t["toc_Ared"], t["toc_Tred"], toc_build, mem))
         self.build_info = info
         return info
+
+    def zero_pivots(self):
+        """Number of factorizations with an exactly zero pivot in each stage
+        (leaf LUs, elim chains, rem chains, interface blocks), from the
+        LAPACK-style info arrays the kernels fill; one device-to-host copy.
+        A nonzero count warns or raises as check_pivots asks: the solve would
+        carry inf/nan."""
+        infos = dict(leaf=self.hps.leaf["info"], elim=self.Aloc_solver.info, rem=self.Ared_solver.info)
+        if self.Tred_solver is not None:
+            infos["interface"] = self.Tred_solver.info
+        counts = torch.stack([(v != 0).sum() for v in infos.values()]).cpu().tolist()
+        out = dict(zip(infos.keys(), (int(c) for c in counts)))
+        bad = {k: v for k, v in out.items() if v}
+        if bad and self.check_pivots != "ignore":
+            msg = "zero pivots in the factorization (factorizations affected per stage: %s)" % bad
+            if self.check_pivots == "raise":
+                raise FloatingPointError(msg)
+            warnings.warn(msg, RuntimeWarning, stacklevel=3)
+        return out
 
     def _dt7(self, DtN):
         """DtN blocks indexed [slab, slab column, leaf row, a, :, b, :] with
@@ -403,13 +436,13 @@ class Domain_Solver:
 
     # ------------------------------------------------------------------ solve
     def _upad(self, rem, ncols):
-        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        Np, n1, B = rem.shape[0], self.Npan_loc, self.rem_bs
         upad = torch.zeros(Np, (n1 + 1) * B, ncols, dtype=rem.dtype, device=rem.device)
         upad[:, B:n1 * B] = rem
         return upad
 
     def _u_local_all(self, upad):
-        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        Np, n1, B = upad.shape[0], self.Npan_loc, self.rem_bs
         nc = upad.shape[-1]
         return torch.as_strided(upad, (Np, n1, 2 * B, nc), (upad.stride(0), B * nc, nc, 1),
                                 upad.storage_offset()).reshape(Np * n1, 2 * B, nc)
@@ -417,7 +450,8 @@ class Domain_Solver:
     def _scatter_rem(self, CT):
         """Sum per-leaf-row [bottom; top] rem contributions (Np*n1, 2B, nc)
         into the rem rows of each slab -> (Np, (n1-1)*B, nc)."""
-        Np, n1, B = self.Npan, self.Npan_loc, self.rem_bs
+        n1, B = self.Npan_loc, self.rem_bs
+        Np = CT.shape[0] // n1
         nc = CT.shape[-1]
         C5 = CT.view(Np, n1, 2, B, nc)
         g = torch.zeros(Np, n1 + 1, B, nc, dtype=CT.dtype, device=CT.device)
@@ -425,82 +459,140 @@ class Domain_Solver:
         g[:, 1:] += C5[:, :, 1]
         return g[:, 1:n1].reshape(Np, (n1 - 1) * B, nc)
 
-    def solve_slab(self, f_C, transpose=False):
-        """A_CC(slab)^{-1} f for every slab; f_C (Npan, |I_C|, nrhs)."""
-        if transpose:
-            raise NotImplementedError("transposed slab solves are not provided")
-        Np, n1, m, f = self.Npan, self.Npan_loc, self.elim_nblocks, self.elim_bs
+    def _slab_solve(self, f_C, transpose=False, s0=0, s1=None):
+        """A_CC(slab)^{-1} f (transpose: A_CC(slab)^{-T} f) for the slabs
+        s0..s1-1 through elim -> rem -> elim (slablu_solver.py:901-943);
+        f_C (s1 - s0, |I_C|, nrhs) in the working precision.  Returns (sol,
+        elim part (rows, m f, nrhs), padded rem part (slabs, (n1+1) B, nrhs)).
+        Plain: t = A_ee^{-1} f_e, u = Ared^{-1}(f_r - A_re t),
+        x_e = t - (A_ee^{-1} A_er) u.  Transposed: t = A_ee^{-T} f_e,
+        u = Ared^{-T}(f_r - A_er^T t), x_e = A_ee^{-T}(f_e - A_re^T u)."""
+        n1, m, f, B = self.Npan_loc, self.elim_nblocks, self.elim_bs, self.rem_bs
+        s1 = self.Npan if s1 is None else s1
+        Np = s1 - s0
+        rows = slice(s0 * n1, s1 * n1)
         ne = n1 * m * f
         nc = f_C.shape[-1]
         fe = f_C[:, :ne].reshape(Np * n1, m * f, nc).contiguous()
         fr = f_C[:, ne:]
-        t = self.Aloc_solver.lusolve_sweep(fe.clone())
-        CT = torch.empty(Np * n1, 2 * self.rem_bs, nc, dtype=t.dtype, device=t.device)
-        hip.gemm(self.cC, t, CT)
+        t = self.Aloc_solver.lusolve_sweep(fe.clone(), b_start=s0 * n1, b_end=s1 * n1, transpose=transpose)
+        CT = torch.empty(Np * n1, 2 * B, nc, dtype=t.dtype, device=t.device)
+        hip.gemm(self.cR[rows].transpose(1, 2) if transpose else self.cC[rows], t, CT)
         g = fr - self._scatter_rem(CT)
         upad = self._upad(g, nc)
-        self.Ared_solver.lusolve_sweep(upad[:, self.rem_bs:n1 * self.rem_bs])
-        ue = t.clone()
-        hip.gemm(self.cW, self._u_local_all(upad), ue, alpha=-1.0, beta=1.0)
+        self.Ared_solver.lusolve_sweep(upad[:, B:n1 * B], b_start=s0, b_end=s1, transpose=transpose)
+        ul = self._u_local_all(upad)
+        if transpose:
+            ue = fe.clone()
+            hip.gemm(self.cC[rows].transpose(1, 2), ul, ue, alpha=-1.0, beta=1.0)
+            self.Aloc_solver.lusolve_sweep(ue, b_start=s0 * n1, b_end=s1 * n1, transpose=True)
+        else:
+            ue = t.clone()
+            hip.gemm(self.cW[rows], ul, ue, alpha=-1.0, beta=1.0)
         sol = torch.empty_like(f_C)
         sol[:, :ne] = ue.view(Np, ne, nc)
-        sol[:, ne:] = upad[:, self.rem_bs:n1 * self.rem_bs]
+        sol[:, ne:] = upad[:, B:n1 * B]
         return sol, ue, upad
 
-    def _AXC(self, ue, upad):
-        """Z_j = A[X_j, C] y for every (slab, leaf row): (Np, n1, 2, f, nc)."""
-        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+    def solve_slab(self, Npan_start, Npan_end, f, transpose=False):
+        """A_CC(slab)^{-1} f (or A_CC(slab)^{-T} f) for the slabs
+        Npan_start..Npan_end-1; f (Npan_end - Npan_start, |I_C|, nrhs), in any
+        dtype and device, returned in f's (slablu_solver.py:900-943)."""
+        fd = torch.as_tensor(f).to(self.device, self.wdt)
+        sol, _, _ = self._slab_solve(fd, transpose, Npan_start, Npan_end)
+        return sol.to(f.device, f.dtype) if torch.is_tensor(f) else sol
+
+    def _AXC(self, ue, upad, transpose=False):
+        """Z_j = A[X_j, C] y for every (slab, leaf row), or (A[C, X_j])^T y
+        with transpose: (Np, n1, 2, f, nc)."""
+        n1, f = self.Npan_loc, self.elim_bs
         nc = ue.shape[-1]
+        Np = ue.shape[0] // n1
+        Ae, Ar = (self.cE.transpose(1, 2), self.cXr.transpose(1, 2)) if transpose else (self.cAXe, self.cAXr)
         Z = torch.empty(Np * n1, 2 * f, nc, dtype=ue.dtype, device=ue.device)
-        hip.gemm(self.cAXe, ue, Z)
-        hip.gemm(self.cAXr, self._u_local_all(upad), Z, beta=1.0)
+        hip.gemm(Ae, ue, Z)
+        hip.gemm(Ar, self._u_local_all(upad), Z, beta=1.0)
         return Z.view(Np, n1, 2, f, nc)
 
-    def _ACX(self, x):
-        """A[C, X] x for every slab from x (Np, n1, 2f, nc) -> (Np, |I_C|, nc)."""
+    def _ACX(self, x, transpose=False):
+        """A[C, X] x for every slab from x (Np, n1, 2f, nc), or (A[X, C])^T x
+        with transpose -> (Np, |I_C|, nc)."""
         Np, n1, m, f = self.Npan, self.Npan_loc, self.elim_nblocks, self.elim_bs
         nc = x.shape[-1]
+        Ae, Ar = (self.cAXe.transpose(1, 2), self.cAXr.transpose(1, 2)) if transpose else (self.cE, self.cXr)
         xe = x.reshape(Np * n1, 2 * f, nc).contiguous()
         e = torch.empty(Np * n1, m * f, nc, dtype=x.dtype, device=x.device)
-        hip.gemm(self.cE, xe, e)
+        hip.gemm(Ae, xe, e)
         r = torch.empty(Np * n1, 2 * self.rem_bs, nc, dtype=x.dtype, device=x.device)
-        hip.gemm(self.cXr, xe, r)
+        hip.gemm(Ar, xe, r)
         out = torch.empty(Np, len(self.I_C), nc, dtype=x.dtype, device=x.device)
         out[:, :n1 * m * f] = e.view(Np, n1 * m * f, nc)
         out[:, n1 * m * f:] = self._scatter_rem(r)
         return out
 
-    def solve_reduced(self, ff):
-        """A_CC x = ff on the face unknowns I_Ctot (slablu_solver.py:543-590);
-        ff (|I_Ctot|, nrhs) in the working precision (interface in FP64)."""
+    def _interface_to_slabs(self, sx, dtype):
+        """Interface values (interfaces, n1, f, nc) -> per-slab [L; R] face
+        values (Np, n1, 2, f, nc) (zero on the Dirichlet sides)."""
+        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+        nc = sx.shape[-1]
+        if self.periodic_bc:
+            x = torch.empty(Np, n1, 2, f, nc, dtype=dtype, device=sx.device)
+            x[:, :, 0] = sx.to(dtype)
+            x[:, :, 1] = torch.roll(sx, -1, dims=0).to(dtype)
+        else:
+            x = torch.zeros(Np, n1, 2, f, nc, dtype=dtype, device=sx.device)
+            x[1:, :, 0] = sx.to(dtype)
+            x[:Np - 1, :, 1] = sx.to(dtype)
+        return x
+
+    def _slabs_to_interface(self, Z):
+        """Per-slab [L; R] contributions (Np, n1, 2, f, nc) summed onto the
+        interfaces -> (interfaces, n1, f, nc) in FP64 (interface j sees slab
+        j-1 from the left and slab j from the right)."""
+        Np = self.Npan
+        if self.periodic_bc:
+            return torch.roll(Z[:, :, 1], 1, dims=0).double() + Z[:, :, 0].double()
+        return Z[:Np - 1, :, 1].double() + Z[1:, :, 0].double()
+
+    def solve_reduced(self, ff, transpose=False):
+        """A_CC x = ff on the face unknowns I_Ctot (slablu_solver.py:543-590),
+        or A_CC^T x = ff with transpose; ff (|I_Ctot|, nrhs) in the working
+        precision (interface in FP64)."""
         Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
         nC = len(self.I_C)
         L = n1 * f
         nc = ff.shape[-1]
         ffC = ff[:Np * nC].view(Np, nC, nc)
-        yC, ue, upad = self.solve_slab(ffC)
+        yC, ue, upad = self._slab_solve(ffC, transpose)
         if Np == 1:
             return yC.reshape(Np * nC, nc)
-        Z = self._AXC(ue, upad)
-        if self.periodic_bc:
-            # interface j sees slab j-1 from the left and slab j from the right
-            ffX = ff[Np * nC:].view(Np, n1, f, nc).to(torch.float64) \
-                - torch.roll(Z[:, :, 1], 1, dims=0).double() - Z[:, :, 0].double()
-            solX = self.Tred_solver.solve(ffX.reshape(Np * L, nc))
-            sx = solX.view(Np, n1, f, nc).to(ff.dtype)
-            x = torch.empty(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
-            x[:, :, 0] = sx
-            x[:, :, 1] = torch.roll(sx, -1, dims=0)
-        else:
-            ffX = ff[Np * nC:].view(Np - 1, n1, f, nc).to(torch.float64) - Z[:Np - 1, :, 1].double() \
-                - Z[1:, :, 0].double()
-            solX = self.Tred_solver.solve(ffX.reshape((Np - 1) * L, nc))
-            sx = solX.view(Np - 1, n1, f, nc)
-            x = torch.zeros(Np, n1, 2, f, nc, dtype=ff.dtype, device=ff.device)
-            x[1:, :, 0] = sx.to(ff.dtype)
-            x[:Np - 1, :, 1] = sx.to(ff.dtype)
-        solC, _, _ = self.solve_slab(ffC - self._ACX(x.view(Np, n1, 2 * f, nc)))
+        nI = Np if self.periodic_bc else Np - 1
+        ffX = ff[Np * nC:].view(nI, n1, f, nc).to(torch.float64) - self._slabs_to_interface(
+            self._AXC(ue, upad, transpose))
+        solX = self.Tred_solver.solve(ffX.reshape(nI * L, nc), transpose=transpose)
+        x = self._interface_to_slabs(solX.view(nI, n1, f, nc), ff.dtype)
+        solC, _, _ = self._slab_solve(ffC - self._ACX(x.view(Np, n1, 2 * f, nc), transpose), transpose)
         return torch.cat((solC.reshape(Np * nC, nc), solX.to(ff.dtype)), dim=0)
+
+    def apply_Tred(self, rhs, transpose=False):
+        """Action of the interface Schur complement T = A_XX - A_XC A_CC^{-1}
+        A_CX (or T^T) on rhs (|I_slabX|, nrhs) (slablu_solver.py:963-1025),
+        through the slab solves; FP64 result."""
+        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+        nI = Np if self.periodic_bc else Np - 1
+        x64 = torch.as_tensor(rhs).to(self.device, torch.float64)
+        nc = x64.shape[-1]
+        sx = x64.view(nI, n1, f, nc)
+        x = self._interface_to_slabs(sx, self.wdt)
+        v = self._ACX(x.view(Np, n1, 2 * f, nc), transpose)
+        _, ue, upad = self._slab_solve(v, transpose)
+        Z = self._slabs_to_interface(self._AXC(ue, upad, transpose))
+        Aii = self.interface_blocks().to(torch.float64)
+        if transpose:
+            Aii = Aii.transpose(1, 2)
+        y = torch.empty(nI, n1 * f, nc, dtype=torch.float64, device=self.device)
+        hip.gemm(Aii, x64.view(nI, n1 * f, nc), y)
+        return (y - Z.reshape(nI, n1 * f, nc)).reshape(nI * n1 * f, nc)
 
     def get_rhs(self, uu_dir_func, ff_body_func=None):
         """ff = -A[Ctot, Xtot] u_dir (slablu_solver.py:489-526) on the device
@@ -628,20 +720,43 @@ class Domain_Solver:
                 c = c * rho / (1.0 - rho)
         return c <= self.refine_tol
 
+    @staticmethod
+    def _progressed(res, res_prev, res0):
+        """Residual guard of the stop test: a step only counts as converged
+        when it still reduced the FP64 residual (by at least half) or the
+        residual is at the FP64 floor (1e-12 of the initial one).  A step
+        that stagnates (a small update because the iteration cannot reduce
+        the residual) makes a small change c_k and a small ratio rho, which
+        the change-based test alone would accept."""
+        if res0 <= 0.0 or res <= 1e-12 * res0:
+            return True
+        return res_prev is not None and res <= 0.5 * res_prev
+
     def _refine_richardson(self, x_face, uc, fb=None):
         """x <- x + M^{-1}(b - A x) until the face correction is small."""
         Ic = self._Ic()
         prev = None
+        res0 = res_prev = None
+        done = False
         for it in range(self.refine_maxit):
             rI, rF = self.full_residual(x_face, uc, fb=fb)
             dI, dC = self._correction(rI, rF)
             x_face[Ic] += dC
             uc += dI
-            rel_corr = (torch.linalg.norm(dC) / torch.linalg.norm(x_face)).item()
+            v = torch.stack((torch.linalg.norm(dC), torch.linalg.norm(x_face),
+                             torch.sqrt(torch.sum(rI * rI) + torch.sum(rF * rF)))).cpu().tolist()
+            rel_corr, res = v[0] / v[1], v[2]
+            res0 = res if res0 is None else res0
             self.refine_history.append(rel_corr)
-            if self._converged(rel_corr, prev):
+            self.refine_residuals.append(res / res0 if res0 > 0 else 0.0)
+            # the residual is that of the iterate before this correction: a
+            # step is judged on the progress of the step before it too
+            if self._converged(rel_corr, prev) and self._progressed(res, res_prev, res0):
+                done = True
                 break
             prev = rel_corr if it >= 1 else None
+            res_prev = res
+        return done
 
     def _graph_step(self, nc):
         """The FGMRES operator step (rI, rF) -> (zI, zC, wI, wF) = (M^{-1} r,
@@ -707,6 +822,8 @@ class Domain_Solver:
         steps = 0
         done = False
         prev = None
+        res0 = None           # per column: norm of the first residual
+        res_prev = None       # largest relative residual estimate of the step before
         while steps < self.refine_maxit and not done:
             rI, rF = self.full_residual(x_face, uc, fb=fb)
             V = torch.empty(nc, kmax + 1, ntot, dtype=torch.float64, device=dev)
@@ -718,6 +835,8 @@ class Domain_Solver:
             # one transfer: beta and |x_face|^2
             bx = torch.cat((beta, (torch.linalg.norm(x_face) ** 2).reshape(1))).cpu().numpy()
             beta, xf2 = bx[:nc], float(bx[nc])
+            if res0 is None:
+                res0 = np.where(beta > 0, beta, 1.0)
             H = np.zeros((nc, kmax + 1, kmax))
             # face parts of the directions: Gram matrix G and g = Z_face^T xc0,
             # so the face change and the face norm of every iterate follow on
@@ -758,20 +877,29 @@ class Domain_Solver:
                 G[:, :j + 1, j] = hg[:, j + 2:2 * j + 3]
                 G[:, j, :j + 1] = hg[:, j + 2:2 * j + 3]
                 gx[:, j] = hg[:, 2 * j + 3]
-                y = np.zeros((nc, j + 1))
-                for col in range(nc):
-                    rhs = np.zeros(j + 2)
-                    rhs[0] = beta[col]
-                    y[col] = np.linalg.lstsq(H[col, :j + 2, :j + 1], rhs, rcond=None)[0]
+                # min ||beta e1 - H y|| for every column at once (batched QR of
+                # the (j+2) x (j+1) Hessenberg matrices); the minimum is the
+                # FP64 residual norm ||b - A x_j|| of the iterate (right
+                # preconditioning), a free residual estimate
+                Hj = H[:, :j + 2, :j + 1]
+                Q, R = np.linalg.qr(Hj)
+                qb = Q[:, 0, :] * beta[:, None]                       # Q^T (beta e1)
+                y = np.linalg.solve(R, qb[:, :, None])[:, :, 0]
+                rhs = np.zeros((nc, j + 2))
+                rhs[:, 0] = beta
+                rres = np.linalg.norm(rhs - np.einsum("cij,cj->ci", Hj, y), axis=1)
+                res_rel = float(np.max(rres / res0))
                 dy = y - np.pad(y_prev, ((0, 0), (0, 1)))
                 Gj = G[:, :j + 1, :j + 1]
                 num2 = float(np.einsum("ci,cij,cj->", dy, Gj, dy))
                 xn2 = xf2 + float(np.einsum("ci,cij,cj->", y, Gj, y)) + 2.0 * float((y * gx[:, :j + 1]).sum())
                 rel = float(np.sqrt(max(num2, 0.0)) / np.sqrt(max(xn2, 1e-300)))
                 self.refine_history.append(rel)
+                self.refine_residuals.append(res_rel)
                 y_prev = y
                 steps += 1
-                done = self._converged(rel, prev)
+                done = self._converged(rel, prev) and self._progressed(res_rel, res_prev, 1.0)
+                res_prev = res_rel
                 # the first step moves from the starting guess, not by a
                 # contraction: rates start from the second step
                 prev = rel if steps >= 2 else None
@@ -785,6 +913,7 @@ class Domain_Solver:
             x_face[Ic] += dC
             uc += dI
             del V, Z
+        return done
 
     def solve_refined(self, uu_dir_func, ff_body_func=None):
         """Mixed precision: FP32 factors, FP64 residuals of the full system,
@@ -803,11 +932,13 @@ class Domain_Solver:
         # un-condensed residual r_I = f - A_loc u
         fb = None if ff_body_func is None else hps.body_load(ff_body_func, nc, torch.float64)
         self.refine_history = []
+        self.refine_residuals = []
         if self.refine_method == "fgmres":
-            self._refine_fgmres(x_face, uc, fb=fb)
+            done = self._refine_fgmres(x_face, uc, fb=fb)
             self._solves_since_build += 1
         else:
-            self._refine_richardson(x_face, uc, fb=fb)
+            done = self._refine_richardson(x_face, uc, fb=fb)
+        self.refine_converged = bool(done)
         uu = torch.empty(hps.nboxes, ops.p * ops.p, nc, dtype=torch.float64, device=dev)
         rI, rF = self.full_residual(x_face, uc, uu=uu, fb=fb)
         torch.cuda.synchronize()
@@ -818,17 +949,50 @@ class Domain_Solver:
             ff0 = ff0 + self._body_reduce(fb)[self._Ic()].double()
         rel = (torch.linalg.norm(rF) / torch.linalg.norm(ff0)).item()
         self.last_face = x_face
+        self.solve_info = dict(refine_converged=self.refine_converged, refine_steps=len(self.refine_history),
+                               refine_history=list(self.refine_history),
+                               refine_residuals=list(self.refine_residuals), res_face=rel)
+        if not done and self.on_unconverged != "ignore":
+            msg = ("mixed-precision refinement stopped after %d steps without meeting refine_tol=%g "
+                   "(last change %.3e, relative residual %.3e)"
+                   % (len(self.refine_history), self.refine_tol,
+                      self.refine_history[-1] if self.refine_history else float("nan"),
+                      self.refine_residuals[-1] if self.refine_residuals else float("nan")))
+            if self.on_unconverged == "raise":
+                raise ArithmeticError(msg)
+            warnings.warn(msg, RuntimeWarning, stacklevel=3)
         return uu.reshape(-1, nc), rel, toc, resloc
 
     # ------------------------------------------------------- self checks
-    def test_slab_solve(self, nrhs=3):
-        """||A_CC(slab) x - f|| / ||f|| for random f, all slabs at once
-        (slablu_solver.py:945-959), using the matrix-free face operator."""
+    def test_slab_solve(self, nrhs=3, transpose=False):
+        """||A_CC(slab) x - f|| / ||f|| (or with A_CC^T) for random f, all
+        slabs at once (slablu_solver.py:945-959), using the matrix-free face
+        operator."""
         Np, nC = self.Npan, len(self.I_C)
         f = torch.rand(Np, nC, nrhs, dtype=self.wdt, device=self.device)
-        sol, _, _ = self.solve_slab(f)
+        sol, _, _ = self._slab_solve(f, transpose)
         xf = torch.zeros(self.hps.nface, nrhs, dtype=torch.float64, device=self.device)
         IsC = self.I_slabC.to(self.device)
         xf[IsC] = sol.reshape(-1, nrhs).double()
-        y = self.hps.apply_A(xf, self.hps.leaf["DtN"].double())[IsC]
+        y = self.hps.apply_A(xf, self.hps.leaf["DtN"].double(), transpose)[IsC]
         return (torch.linalg.norm(y - f.reshape(-1, nrhs).double()) / torch.linalg.norm(f.double())).item()
+
+    def test_Tred_solver(self, nrhs=3, transpose=False):
+        """||T x - f|| / ||f|| for the interface solve of random f, T applied
+        through the slab solves (apply_Tred, slablu_solver.py:689-695)."""
+        nX = self.I_slabX.shape[0]
+        f = torch.rand(nX, nrhs, dtype=torch.float64, device=self.device)
+        x = self.Tred_solver.solve(f, transpose=transpose)
+        y = self.apply_Tred(x, transpose=transpose)
+        return (torch.linalg.norm(y - f) / torch.linalg.norm(f)).item()
+
+    def test_solve_reduced(self, nrhs=3, transpose=False):
+        """||A_CC x - f|| / ||f|| (or with A_CC^T) of the two-level solve for
+        random f on all face unknowns I_Ctot."""
+        Ic = self._Ic()
+        f = torch.rand(Ic.shape[0], nrhs, dtype=torch.float64, device=self.device)
+        x = self.solve_reduced(f.to(self.wdt), transpose=transpose).double()
+        xf = torch.zeros(self.hps.nface, nrhs, dtype=torch.float64, device=self.device)
+        xf[Ic] = x
+        y = self.hps.apply_A(xf, self.hps.leaf["DtN"].double(), transpose)[Ic]
+        return (torch.linalg.norm(y - f) / torch.linalg.norm(f)).item()

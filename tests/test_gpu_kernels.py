@@ -289,9 +289,8 @@ def _btri_dense(diag, sub, sup):
 @pytest.mark.parametrize("twisted", [False, True])
 def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
     """Block-tridiagonal LU factor + sweeps (sequential, or twisted: both
-    chain ends towards block m//2) against a dense solve."""
-    if twisted and transpose:
-        pytest.skip("twisted factors have no transposed sweep")
+    chain ends towards block m//2), plain and transposed, against a dense
+    solve."""
     hip = _hip()
     g = torch.Generator().manual_seed(b * 100 + m * 10 + bs)
     diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
@@ -313,10 +312,11 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (2, 2, 37), (2, 11, 100), (3, 4, 100)])
 @pytest.mark.parametrize("nrhs", [1, 3, 32, 70])
-def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs):
+@pytest.mark.parametrize("transpose", [False, True])
+def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs, transpose):
     """Twisted block LU whose diagonal blocks are applied through their
     triangular inverses (L^{-1} P, U^{-1}): the same solution as the
-    substitution sweeps and as a dense solve."""
+    substitution sweeps and as a dense solve, plain and transposed."""
     hip = _hip()
     g = torch.Generator().manual_seed(b * 1000 + m * 100 + bs + nrhs)
     diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
@@ -327,14 +327,14 @@ def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs):
     ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
     hip.btri_factor(diag, sub, sup, ipiv, twisted=True)
     f_lu = f0.to("cuda", dtype).contiguous()
-    hip.btri_solve(diag, sub, sup, ipiv, f_lu, twisted=True)
+    hip.btri_solve(diag, sub, sup, ipiv, f_lu, twisted=True, transpose=transpose)
     uinv = diag.clone()
     linv = torch.empty_like(uinv)
     hip.btri_invert_diag(uinv, ipiv, linv)
     f = f0.to("cuda", dtype).contiguous()
-    hip.btri_solve(uinv, sub, sup, None, f, twisted=True, linv=linv)
+    hip.btri_solve(uinv, sub, sup, None, f, twisted=True, linv=linv, transpose=transpose)
     M = _btri_dense(diag0, sub0[:, :m - 1], sup0[:, :m - 1])
-    ref = torch.linalg.solve(M, f0)
+    ref = torch.linalg.solve(M.transpose(1, 2) if transpose else M, f0)
     torch.cuda.synchronize()
     assert _rel(f, ref) < 200 * TOL[dtype]
     assert _rel(f, f_lu.double().cpu()) < 200 * TOL[dtype]
@@ -347,7 +347,8 @@ def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("twisted", [False, True])
 @pytest.mark.parametrize("nrhs", [1, 70])
-def test_btri_solve_block_sparse_rhs(dtype, twisted, nrhs):
+@pytest.mark.parametrize("transpose", [False, True])
+def test_btri_solve_block_sparse_rhs(dtype, twisted, nrhs, transpose):
     """Right-hand sides whose leading blocks are zero (a different first
     nonzero block per column, one all-zero 32-column tile): the forward
     sweeps start at the first nonzero block and must give the dense
@@ -370,8 +371,9 @@ def test_btri_solve_block_sparse_rhs(dtype, twisted, nrhs):
     ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device="cuda")
     hip.btri_factor(diag, sub, sup, ipiv, twisted=twisted)
     f = f0.to("cuda", dtype).contiguous()
-    hip.btri_solve(diag, sub, sup, ipiv, f, twisted=twisted)
-    ref = torch.linalg.solve(_btri_dense(diag0, sub0, sup0), f0)
+    hip.btri_solve(diag, sub, sup, ipiv, f, twisted=twisted, transpose=transpose)
+    M = _btri_dense(diag0, sub0, sup0)
+    ref = torch.linalg.solve(M.transpose(1, 2) if transpose else M, f0)
     torch.cuda.synchronize()
     assert _rel(f, ref) < 200 * TOL[dtype]
     if nrhs > 64:
@@ -529,7 +531,7 @@ def test_block_tridiagonal_api(form, twisted):
     x = T.lusolve_sweep(f0[2:5].cuda().clone(), b_start=2, b_end=5)
     torch.cuda.synchronize()
     assert _rel(x, torch.linalg.solve(M[2:5], f0[2:5])) < 1e-11
-    if form == "lu" and not T.twisted:
+    if form == "lu":
         xt = T.lusolve_sweep(f0.cuda().clone(), transpose=True)
         torch.cuda.synchronize()
         assert _rel(xt, torch.linalg.solve(M.transpose(1, 2), f0)) < 1e-11

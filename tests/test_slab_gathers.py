@@ -114,3 +114,16 @@ def test_refinement_stop_test_uses_the_geometric_tail():
     assert not conv(2.7e-7, 1.0e-6)               # rho 0.27: too slow to trust the tail
     assert not conv(5e-7, 2e-5)                   # rho 0.025: tail 1.28e-8 > tol
     assert conv(9e-9, 1e-8)                       # slow but already below tol
+
+
+def test_refinement_residual_guard():
+    """A step passes the stop test only if it still halved the FP64 residual
+    (or the residual is at the 1e-12 floor): a stagnating step, whose tiny
+    update gives a tiny change, is not taken as converged."""
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    prog = Domain_Solver._progressed
+    assert prog(1e-6, 1e-4, 1.0)                  # contracting
+    assert not prog(0.9e-4, 1e-4, 1.0)            # stagnating
+    assert not prog(1e-6, None, 1.0)              # no previous step
+    assert prog(5e-13, 5e-13, 1.0)                # at the FP64 floor
+    assert prog(0.0, None, 0.0)                   # zero right-hand side
