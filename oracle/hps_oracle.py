@@ -806,6 +806,70 @@ def setup_problem(n, p, ppw=None, kh=None, domain=((0.0, 1.0), (0.0, 1.0)), bfie
     return hps, part, kh
 
 
+# ----------------------------------------------------------------------------
+# 7. Adjoint of the FP64 solve with respect to a per-node coefficient
+#    (the north star's adjoint form; the reference has the transposed slab
+#    sweeps it needs, block_tridiag_solver_dense.py:205-222 and
+#    slablu_solver.py:900-943, but no gradient)
+# ----------------------------------------------------------------------------
+
+
+def helmholtz_nodes_solve(hps, part, kh, b_nodes, uu_dir_func):
+    """-Lap u - kh^2 b u = 0 with b given per leaf node (nboxes, p^2):
+    exact discrete solution at every leaf node (nboxes*p^2,)."""
+    hps.c_nodes = -(kh ** 2) * np.asarray(b_nodes, dtype=np.float64).reshape(hps.nboxes, hps.p ** 2)
+    r = full_solve(hps, part, uu_dir_func, known_sol=False)
+    return r["sol_tot"][:, 0], r
+
+
+def adjoint_gradient(hps, part, kh, b_nodes, uu_dir_func, obs, d):
+    """dL/db for L = 1/2 ||u[obs] - d||^2 (obs: flat node indices, d their
+    data), u from helmholtz_nodes_solve.  The un-condensed system is
+    F = [A_cc u_c + A_cx u_x ; sum_b P_b^T (N_c u_c + N_x u_x)] = 0 over the
+    leaf interiors and the face unknowns I_Ctot, with c = -kh^2 b only on the
+    diagonal of A_cc (interior rows), so dF/db_i = -kh^2 u_i e_i and
+    dL/db_i = kh^2 lam_i u_i at interior nodes, lam from F_u^T lam = dL/du:
+      A_CC^T lam_C = g_C - (sum_b P_b^T S_b^T g_c)[I_Ctot],  S = A_cc^{-1} A_cx,
+      lam_c = A_cc^{-T} (g_c - N_c^T lam_x)            (per leaf).
+    The node gradient reaches the face unknowns through the corner-inclusive
+    interpolation Interp_mat (face and corner values of the leaf nodes).
+    Returns (gradient (nboxes*p^2,), loss, u)."""
+    p = hps.p
+    u, r = helmholtz_nodes_solve(hps, part, kh, b_nodes, uu_dir_func)
+    A = r["A"]
+    nb = hps.nboxes
+    Jc, Jx, Jxr = hps.JJ["Jc"], hps.JJ["Jx"], hps.JJ["Jxreorder"]
+    res = u[obs] - d
+    loss = 0.5 * float(res @ res)
+    g = np.zeros(nb * p * p)
+    np.add.at(g, obs, res)
+    g = g.reshape(nb, p * p)
+    g_c = g[:, Jc]
+    g_bnd = hps.Interp_mat.T @ g[:, Jxr].T                      # (4(p-2), nb)
+    I = hps.iext_box()
+    nface = A.shape[0]
+    g_face = np.zeros(nface)
+    np.add.at(g_face, I.ravel(), g_bnd.T.ravel())
+    Aall = hps.aloc(np.arange(nb))
+    Acc = Aall[:, Jc][:, :, Jc]
+    Acx = Aall[:, Jc][:, :, Jx]
+    S = np.linalg.solve(Acc, Acx)                                # (nb, ni, nx)
+    z = np.einsum("bix,bi->bx", S, g_c)
+    np.add.at(g_face, I.ravel(), -z.ravel())
+    A_CC = A[part.I_Ctot][:, part.I_Ctot].tocsc()
+    lam_C = spla.splu(A_CC).solve(g_face[part.I_Ctot], trans="T")
+    lam_face = np.zeros(nface)
+    lam_face[part.I_Ctot] = lam_C
+    lam_x = lam_face[I]                                          # (nb, nx)
+    N_c = hps.Nx[:, Jc]
+    rhs = g_c - lam_x @ N_c
+    lam_c = np.linalg.solve(np.transpose(Acc, (0, 2, 1)), rhs[:, :, None])[:, :, 0]
+    u_c = u.reshape(nb, p * p)[:, Jc]
+    grad = np.zeros((nb, p * p))
+    grad[:, Jc] = (kh ** 2) * lam_c * u_c
+    return grad.ravel(), loss, u
+
+
 def uu_periodic_mode(xx, kh, m=2):
     """cos(2 pi m x) cos(k_y y), k_y^2 = kh^2 - (2 pi m)^2: an exact solution
     of -Lap u - kh^2 u = 0 that is 1-periodic in x (a closed form for the

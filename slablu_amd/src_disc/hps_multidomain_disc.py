@@ -113,20 +113,69 @@ class HPS_Multidomain:
             self.xx_flat_d = self.grid_xx.reshape(-1, 2).to(dev)
         return dev
 
-    def factor_leaves(self, device=None, dtype=torch.float64):
+    def factor_leaves(self, device=None, dtype=torch.float64, chunk=None, store=True):
         """Evaluate the PDE coefficients at every node and condense every
-        leaf (one batched pass); keeps the factors."""
+        leaf.  chunk: leaves per batch (None: all in one batch; the
+        reference sizes its chunks from free memory,
+        hps_parallel_leaf_ops.py:102-141).  store=False keeps only the DtN
+        maps: the leaf factors are recomputed, chunk by chunk, wherever they
+        are needed again (leaf recovery, body loads, adjoint), as the
+        reference's solve re-factors every leaf (hps_parallel_leaf_ops.py:
+        73-92)."""
         dev = self._setup_device(device)
         self.leaf = None
         self.coefs = coef_arrays(self.pdo, self.xx_flat_d)
+        nb, ni, nx = self.nboxes, self.ops.ni, self.ops.nx
+        chunk = nb if chunk is None else max(1, min(int(chunk), nb))
         ev = self.leaf_event_pair
         if ev is not None:
             ev[0].record()
-        DtN, M, ipiv, info = self.ops.factor(self.coefs, self.nboxes, dtype)
+        if chunk >= nb and store:
+            DtN, M, ipiv, info = self.ops.factor(self.coefs, nb, dtype)
+        else:
+            DtN = torch.empty(nb, nx, nx, dtype=dtype, device=dev)
+            info = torch.zeros(nb, dtype=torch.int32, device=dev)
+            M = torch.empty(nb, ni, ni + nx, dtype=dtype, device=dev) if store else None
+            ipiv = torch.empty(nb, ni, dtype=torch.int32, device=dev) if store else None
+            Mc = None if store else torch.empty(chunk, ni, ni + nx, dtype=dtype, device=dev)
+            ipc = None if store else torch.empty(chunk, ni, dtype=torch.int32, device=dev)
+            for s in range(0, nb, chunk):
+                e = min(s + chunk, nb)
+                out = (DtN[s:e], M[s:e] if store else Mc[:e - s], ipiv[s:e] if store else ipc[:e - s], info[s:e])
+                self.ops.factor(self._coef_slice(s, e), e - s, dtype, out=out)
+            del Mc, ipc
         if ev is not None:
             ev[1].record()
-        self.leaf = dict(DtN=DtN, M=M, ipiv=ipiv, perm=hip.piv_to_perm(ipiv), info=info, dtype=dtype)
+        self.leaf = dict(DtN=DtN, M=M, ipiv=ipiv, perm=hip.piv_to_perm(ipiv) if store else None, info=info,
+                         dtype=dtype, chunk=chunk, stored=store)
         return DtN
+
+    def _coef_slice(self, s, e):
+        """Coefficient arrays of leaves s..e-1."""
+        pp = self.p * self.p
+        return {k: (None if v is None else v[s * pp:e * pp]) for k, v in self.coefs.items()}
+
+    def leaf_chunks(self):
+        """(s, e, M, ipiv, perm) over the leaves in the chunks of the
+        factorization: slices of the kept factors, or the factors of leaves
+        s..e-1 recomputed when only the DtN maps are kept (bit-identical:
+        every leaf is factored independently)."""
+        L = self.leaf
+        nb, ni, nx = self.nboxes, self.ops.ni, self.ops.nx
+        if L["stored"]:
+            yield 0, nb, L["M"], L["ipiv"], L["perm"]
+            return
+        chunk = L["chunk"]
+        dev = self._dev
+        Mc = torch.empty(chunk, ni, ni + nx, dtype=L["dtype"], device=dev)
+        Dc = torch.empty(chunk, nx, nx, dtype=L["dtype"], device=dev)
+        ipc = torch.empty(chunk, ni, dtype=torch.int32, device=dev)
+        infc = torch.empty(chunk, dtype=torch.int32, device=dev)
+        for s in range(0, nb, chunk):
+            e = min(s + chunk, nb)
+            k = e - s
+            self.ops.factor(self._coef_slice(s, e), k, L["dtype"], out=(Dc[:k], Mc[:k], ipc[:k], infc[:k]))
+            yield s, e, Mc[:k], ipc[:k], hip.piv_to_perm(ipc[:k])
 
     def get_DtNs(self, device=None, mode="build", data=0, ff_body_func=None):
         """mode 'build': (nboxes, 4(p-2), 4(p-2)) leaf DtN maps (device tensor).
@@ -152,7 +201,12 @@ class HPS_Multidomain:
         L = self.leaf
         u_bnd = u_bnd.to(L["dtype"]).contiguous()
         fb = None if ff_body_func is None else self.body_load(ff_body_func, u_bnd.shape[-1], L["dtype"])
-        uc = self.ops.interior(L["M"], L["ipiv"], u_bnd, fb, perm=L.get("perm"))
+        if L["stored"]:
+            uc = self.ops.interior(L["M"], L["ipiv"], u_bnd, fb, perm=L.get("perm"))
+        else:
+            uc = torch.empty(self.nboxes, self.ops.ni, u_bnd.shape[-1], dtype=L["dtype"], device=u_bnd.device)
+            for s, e, M, ipiv, perm in self.leaf_chunks():
+                uc[s:e] = self.ops.interior(M, ipiv, u_bnd[s:e], None if fb is None else fb[s:e], perm=perm)
         uu = self.ops.assemble_nodes(u_bnd, uc)
         Au = self.ops.apply_aloc(uu, self.coefs)
         res = torch.zeros_like(uu)

@@ -76,20 +76,31 @@ class LeafOps:
         return self._typed[dtype]
 
     # -- build ---------------------------------------------------------------
-    def factor(self, coefs, nboxes, dtype=torch.float64):
+    def factor(self, coefs, nboxes, dtype=torch.float64, out=None):
         """Condense every leaf.  coefs: dict name -> FP64 (nboxes*p*p) device
         tensor.  Returns (DtN (nboxes, nx, nx), M (nboxes, ni, ni+nx) factored,
-        ipiv (nboxes, ni) int32, info (nboxes,) int32)."""
+        ipiv (nboxes, ni) int32, info (nboxes,) int32).  out: optional
+        (DtN, M, ipiv, info) tensors (contiguous, those shapes) written in
+        place, e.g. slices of the whole stack for one chunk of leaves."""
         ni, nx = self.ni, self.nx
-        ipiv = torch.empty(nboxes, ni, dtype=torch.int32, device=self.device)
-        info = torch.zeros(nboxes, dtype=torch.int32, device=self.device)
+        if out is not None:
+            DtN_o, M_o, ipiv, info = out
+            info.zero_()
+        else:
+            DtN_o = M_o = None
+            ipiv = torch.empty(nboxes, ni, dtype=torch.int32, device=self.device)
+            info = torch.zeros(nboxes, dtype=torch.int32, device=self.device)
         ty = self.typed(dtype)
         if coefs.get("c12") is None:
-            M = torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
+            M = M_o if M_o is not None else torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
             hip.leaf_assemble(M, self.p, 0, self.D, self.Dsq, coefs, self.Nxtot)
             hip.getrf(M, ipiv, nelim=ni, npiv=ni, info=info)
             hip.trsm(M[:, :, :ni], M[:, :, ni:], lower=False, unit=False)
-            DtN = ty["Nx_x"].expand(nboxes, nx, nx).contiguous()
+            if DtN_o is not None:
+                DtN = DtN_o
+                DtN.copy_(ty["Nx_x"].expand(nboxes, nx, nx))
+            else:
+                DtN = ty["Nx_x"].expand(nboxes, nx, nx).contiguous()
         else:
             # the cross term couples interior and corner nodes: condense
             # against the corner-inclusive exterior nodes (Jxreorder) and
@@ -101,11 +112,15 @@ class LeafOps:
             hip.leaf_assemble_nodes(Mr, self.p, self.res_tables["Jxreo"], self.D, self.Dsq, coefs)
             hip.getrf(Mr, ipiv, nelim=ni, npiv=ni, info=info)
             hip.trsm(Mr[:, :, :ni], Mr[:, :, ni:], lower=False, unit=False)
-            M = torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
+            M = M_o if M_o is not None else torch.empty(nboxes, ni, ni + nx, dtype=dtype, device=self.device)
             M[:, :, :ni] = Mr[:, :, :ni]
             hip.gemm(Mr[:, :, ni:], ty["Intmap"], M[:, :, ni:])
             del Mr
-            DtN = ty["NxI"].expand(nboxes, nx, nx).contiguous()
+            if DtN_o is not None:
+                DtN = DtN_o
+                DtN.copy_(ty["NxI"].expand(nboxes, nx, nx))
+            else:
+                DtN = ty["NxI"].expand(nboxes, nx, nx).contiguous()
         hip.gemm(ty["Nx_c"], M[:, :, ni:], DtN, alpha=-1.0, beta=1.0)
         return DtN, M, ipiv, info
 

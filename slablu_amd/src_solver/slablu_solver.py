@@ -74,7 +74,7 @@ def _blockdiag(blocks):
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
                  device=None, refine_tol=1e-8, refine_maxit=12, refine_method="fgmres", refine_restart=10,
-                 check_pivots="warn", on_unconverged="warn"):
+                 check_pivots="warn", on_unconverged="warn", leaf_memory_budget=None, leaf_chunk=None):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
         if precision not in ("fp64", "mixed"):
@@ -98,6 +98,12 @@ class Domain_Solver:
         self.refine_converged = None
         self.refine_history, self.refine_residuals = [], []
         self.solve_info = {}
+        # leaf stage under a memory budget (bytes for the kept leaf factors):
+        # when all factors do not fit, only the DtN maps are kept and the
+        # leaves are factored in chunks that fit, again wherever the factors
+        # are needed (slablu_solver.py:781-790 sizes its chunks from free
+        # memory); leaf_chunk forces a chunk size
+        self.leaf_memory_budget, self.leaf_chunk = leaf_memory_budget, leaf_chunk
         # stage timings of build() from events on the stream (no host
         # synchronization between stages)
         self.stage_sync = True
@@ -197,10 +203,11 @@ class Domain_Solver:
                 ev[name].record()
 
         self._mark = mark
+        store, chunk = self._leaf_plan()
         torch.cuda.synchronize()
         tic0 = time.time()
         mark("start")
-        self.hps.factor_leaves(self.device, self.wdt)
+        self.hps.factor_leaves(self.device, self.wdt, chunk=chunk, store=store)
         mark("dtn")
         self.Dt7 = self._dt7(self.hps.leaf["DtN"])
         self.build_Aloc_solver()
@@ -221,8 +228,10 @@ class Domain_Solver:
         else:
             t.update(toc_DtN=float("nan"), toc_Aloc=float("nan"), toc_Ared=float("nan"), toc_Tred=float("nan"),
                      toc_Tform=float("nan"))
-        mem = (self.hps.leaf["M"].numel() * self.hps.leaf["M"].element_size() + self.Aloc_solver.nbytes +
-               self.Ared_solver.nbytes + (self.Tred_solver.nbytes if self.Tred_solver is not None else 0)) / 1e9
+        M = self.hps.leaf["M"]
+        mem = ((M.numel() * M.element_size() if M is not None else 0) + self.hps.leaf["DtN"].numel() *
+               self.hps.leaf["DtN"].element_size() + self.Aloc_solver.nbytes + self.Ared_solver.nbytes +
+               (self.Tred_solver.nbytes if self.Tred_solver is not None else 0)) / 1e9
         info = dict(toc_build=toc_build, mem_build=mem, toc_assembly=t["toc_DtN"], **t)
         info["zero_pivots"] = self.zero_pivots()
         if verbose:
@@ -230,6 +239,22 @@ class Domain_Solver:
                   % (self.precision, t["toc_DtN"], t["toc_Aloc"], t["toc_Ared"], t["toc_Tred"], toc_build, mem))
         self.build_info = info
         return info
+
+    def _leaf_plan(self):
+        """(keep the leaf factors?, leaves per chunk) under leaf_memory_budget."""
+        ni, nx = (self.hps.p - 2) ** 2, 4 * (self.hps.p - 2)
+        per_leaf = ni * (ni + nx) * torch.tensor([], dtype=self.wdt).element_size()
+        nb = self.hps.nboxes
+        budget = self.leaf_memory_budget
+        if budget is None or nb * per_leaf <= budget:
+            return True, self.leaf_chunk
+        if self.precision == "mixed":
+            raise ValueError("the mixed-precision refinement reuses the leaf factors in every step: "
+                             "leaf_memory_budget (%d bytes) must hold them (%d bytes)" % (budget, nb * per_leaf))
+        chunk = max(1, int(budget // per_leaf))
+        if self.leaf_chunk is not None:
+            chunk = min(chunk, self.leaf_chunk)
+        return False, chunk
 
     def zero_pivots(self):
         """Number of factorizations with an exactly zero pivot in each stage
@@ -640,7 +665,8 @@ class Domain_Solver:
         ops, lf = self.hps.ops, self.hps.leaf
         ni = ops.ni
         w = fb.to(lf["dtype"]).contiguous().clone()
-        hip.getrs(lf["M"][:, :, :ni], lf["ipiv"], w, perm=lf.get("perm"))
+        for s, e, M, ipiv, perm in self.hps.leaf_chunks():
+            hip.getrs(M[:, :, :ni], ipiv, w[s:e], perm=perm)
         z = torch.empty(fb.shape[0], ops.nx, fb.shape[-1], dtype=w.dtype, device=w.device)
         hip.gemm(ops.typed(w.dtype)["Nx_c"], w, z, alpha=-1.0)
         y = torch.zeros(self.hps.nface, fb.shape[-1], dtype=w.dtype, device=w.device)
