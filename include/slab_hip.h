@@ -75,6 +75,20 @@ size_t slab_getri_ws(int dtype, int batch, int n);
 int slab_getri(int dtype, int batch, int n, const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv,
                int64_t p_bs, void* X, int64_t ldx, int64_t x_bs, void* ws, void* stream);
 
+/* slab_getri with no allocation and no workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops); it has no counterpart in the reference.  The workspace is a
+ * file-scope device array of SLAB_GETRI_NOALLOC_MAX_BYTES (40 MiB) that each
+ * device holds for the life of the process: problems whose slab_getri_ws
+ * exceeds it are rejected with -2 before any launch (e.g. FP64: one 1000 x
+ * 1000 matrix fits, two do not).  Every call on one device shares that
+ * device's array, so calls that overlap on one device (concurrent streams,
+ * threads) race; calls in order on one stream do not.  Otherwise as
+ * slab_getri (same results, bit for bit). */
+#define SLAB_GETRI_NOALLOC_MAX_BYTES (40u * 1024u * 1024u)
+int slab_getri_noalloc(int dtype, int batch, int n, const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv,
+                       int64_t p_bs, void* X, int64_t ldx, int64_t x_bs, void* stream);
+
 /* perm[r] (0-based) = the row of B that the interchanges ipiv (1-based, as
  * slab_getrf writes them) bring to row r; batched, perm row stride q_bs.
  * Lets repeated solves with one factorization skip replaying the swaps. */

@@ -38,6 +38,7 @@ def _load():
         "slab_piv_to_perm": ([_int, _int, _vp, _i64, _vp, _i64, _vp], _int),
         "slab_getri_ws": ([_int, _int, _int], ctypes.c_size_t),
         "slab_getri": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp], _int),
+        "slab_getri_noalloc": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_getrs_perm": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_btri_factor": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64,
                               _i64, _vp, _vp], _int),
@@ -202,6 +203,28 @@ def getri(LU, ipiv, X=None):
     ws = torch.empty(int(LIB.slab_getri_ws(_dt(LU), b, n)), dtype=torch.uint8, device=LU.device)
     _check(LIB.slab_getri(_dt(LU), b, n, _p(LU), LU.stride(1), LU.stride(0), _p(ipiv), ipiv.stride(0), _p(X),
                           X.stride(1), X.stride(0), _p(ws), _stream()), "slab_getri")
+    return X
+
+
+GETRI_NOALLOC_MAX_BYTES = 40 * 1024 * 1024
+
+
+def getri_noalloc(LU, ipiv, X):
+    """getri into a caller-provided X with no device allocation: the
+    workspace is the library's static per-device scratch array (at most
+    GETRI_NOALLOC_MAX_BYTES; larger problems raise ValueError).  Calls that
+    overlap on one device race on that array."""
+    _require(LU.dim() == 3 and LU.stride(2) == 1 and LU.shape[1] == LU.shape[2], "getri expects square row-major LU")
+    b, n, _ = LU.shape
+    _require(ipiv.dtype == torch.int32 and ipiv.shape[0] == b and ipiv.shape[1] >= n and ipiv.stride(-1) == 1,
+             "ipiv shape/dtype")
+    _require(X.shape == LU.shape and X.dtype == LU.dtype and X.stride(2) == 1, "getri output layout")
+    rc = LIB.slab_getri_noalloc(_dt(LU), b, n, _p(LU), LU.stride(1), LU.stride(0), _p(ipiv), ipiv.stride(0), _p(X),
+                                X.stride(1), X.stride(0), _stream())
+    if rc == -2:
+        raise ValueError("slab_getri_noalloc: %d x %d x %d needs %d bytes of workspace, more than the %d of its "
+                         "static scratch" % (b, n, n, int(LIB.slab_getri_ws(_dt(LU), b, n)), GETRI_NOALLOC_MAX_BYTES))
+    _check(rc, "slab_getri_noalloc")
     return X
 
 
