@@ -2,24 +2,31 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1000] [--p 22]
 
-Workload (default): Helmholtz -Lap u - kh^2 (1 - b(x)) u = 0 on the unit
+Headline workload: Helmholtz -Lap u - kh^2 (1 - b(x)) u = 0 on the unit
 square, n=1000, p=22 (50 x 50 leaves, N = 1,102,000 DOFs), 10 points per
 wavelength (kh = 630.31), b(x) = Gaussian bumps (amplitude 1, width 0.005)
-on a 0.04 lattice with one row and one column removed (L-shaped waveguide),
-Dirichlet data from the free-space Green's function.  A step is one full
-factor + solve of the mixed-precision path (FP32 leaf/slab stages, FP64
-interface LU, FP64-residual refinement of the full discrete system until the face
-solution changes by less than 1e-8, relative).  The FP64 path is timed on the
-same problem and its solution is the reference for the error of the mixed
-solution.
+on a 0.04 lattice with one half row and one half column removed (L-shaped
+waveguide), Dirichlet data from the free-space Green's function of a source
+at (1.5, 1.5).  A step is one full factor + solve of the mixed-precision
+path (FP32 leaf/slab stages, FP64 interface LU, FP64-residual refinement of
+the full discrete system to 1e-8); value = N / (factor + solve) in DOF/s.
 
-Prints one JSON line (metric, value = DOF/s, ...).  roofline: the leaf
-interior solve, run in every refinement step of the mixed path
-(k_getrs_wave, HBM-bound: it streams every leaf's LU factors once per
-call), timed with HIP events on the stream it runs on against its
-algorithmic bytes; traffic is the measured HBM bytes per launch recorded in
-profiles/leaf_solve_traffic.json when present.  cpu_baseline: the oracle on
-a bounded sample of the same workload on the host cores.
+Beside it, in the same JSON line:
+  fp64          the plain FP64 path on the same problem; its solution is the
+                reference for the error of the mixed one (the run fails when
+                that error exceeds 1e-8 or the refinement did not converge);
+  cfg2_fp64     the constant-coefficient problem, FP64, same size;
+  adjoint       34 x 34 leaves, p=16 (N = 259,896): forward (factor + solve)
+                and gradient (adjoint) times of the per-node coefficient;
+  stages        per-stage times with their algorithmic work (fixed formulas)
+                and the fraction of the MI355X peak they reach;
+  roofline      the dominant kernel of the mixed step, the rem-chain block LU
+                (k_btri_lu_factor<float, RT>, twisted: a chain launch and a
+                middle-block launch per factorization), timed with HIP events
+                on its stream over a re-run of the rem factorization of this
+                build; traffic from profiles/rem_factor_traffic.json when present;
+  cpu_baseline  the oracle (numpy/scipy FP64) on a bounded sample of the same
+                workload, factor and solve phases, extrapolated.
 """
 import argparse
 import json
@@ -33,51 +40,25 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor+solve DOF/s at N≈1M p=22: mixed+refined vs FP64 at 1e-8 rel err"
 PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+SRC = (1.5, 1.5)
 
 
-def leaf_solve_bytes(nboxes, ni, itemsize, nrhs=1):
-    """Algorithmic HBM bytes of one leaf interior solve launch: every leaf's
-    LU factors (ni x ni) once, the permutation, and the right-hand side read
-    and written."""
-    return nboxes * (ni * ni * itemsize + ni * 4 + 2 * ni * nrhs * itemsize)
+def algorithmic_work(p, n0, n1, buf, Npan):
+    """Per-stage algorithmic flops of the factorization (fixed formulas:
+    n_i = (p-2)^2, n_b = 4(p-2), L = n1 (p-2), B = buf (p-2))."""
+    f = p - 2
+    ni, nb_, L, B = f * f, 4 * f, n1 * f, buf * f
+    nboxes = n0 * n1
+    return dict(
+        leaf=nboxes * ((2 / 3) * ni ** 3 + 2 * ni ** 2 * nb_ + 2 * nb_ ** 2 * (ni + nb_)),
+        rem_factor=Npan * (n1 - 1) * (14 / 3) * B ** 3,
+        tslab=Npan * (n1 - 1) * 6 * B ** 2 * 2 * L,
+        interface=(Npan - 1) * (14 / 3) * L ** 3,
+    )
 
 
-def time_leaf_solve(dom, reps=20):
-    """Average duration (ms) of one leaf interior solve launch on the factored
-    leaves of dom, with HIP events on the current stream."""
-    import torch
-    from slablu_amd import hip
-    lf = dom.hps.leaf
-    M = lf["M"]
-    ni = M.shape[1]
-    LU = M[:, :, :ni]
-    x0 = torch.randn(M.shape[0], ni, 1, dtype=M.dtype, device=M.device)
-    x = x0.clone()
-    hip.getrs(LU, lf["ipiv"], x, perm=lf["perm"])
-    ms = []
-    for _ in range(reps):
-        x.copy_(x0)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        hip.getrs(LU, lf["ipiv"], x, perm=lf["perm"])
-        e1.record()
-        torch.cuda.synchronize()
-        ms.append(e0.elapsed_time(e1))
-    return sum(ms) / len(ms), M.shape[0], ni, M.element_size()
-
-
-def measured_traffic():
-    """HBM bytes per leaf-solve launch from profiles/leaf_solve_traffic.json
-    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "leaf_solve_traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        d = json.load(fh)
-    return d.get("hbm_bytes_per_launch")
-
-
-def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.0, pde="waveguide"):
+def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.0, pde="waveguide", center=SRC):
     import torch
     from slablu_amd.src_disc import pdo, built_in_funcs as bif
     from slablu_amd.src_solver.slablu_solver import Domain_Solver
@@ -88,23 +69,18 @@ def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.
     a = 1 / (2 * (n / (p - 2)))
     dom = Domain_Solver(torch.tensor([[0, box_x], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
                         precision=precision, periodic_bc=periodic)
-    uu_dir = lambda xx: bif.uu_dir_func_greens(zmap(torch.as_tensor(xx)), kh)
+    uu_dir = lambda xx: bif.uu_dir_func_greens(zmap(torch.as_tensor(xx)), kh, center=center)
     return dom, uu_dir, kh
 
 
-def run_path(dom, uu_dir, steps, warmup, leaf_events):
+def run_path(dom, uu_dir, steps, warmup):
     import torch
     ts, tf, tsol = [], [], []
-    leaf_ms = []
     stages = {}
     sol = None
     for it in range(warmup + steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if leaf_events is not None and it >= warmup:
-            dom.hps.leaf_event_pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-        else:
-            dom.hps.leaf_event_pair = None
         info = dom.build(verbose=False)
         sol, rel, _, _, toc = dom.solve(uu_dir)
         torch.cuda.synchronize()
@@ -115,33 +91,135 @@ def run_path(dom, uu_dir, steps, warmup, leaf_events):
             tsol.append(toc)
             for k in ("toc_DtN", "toc_Aloc", "toc_Ared", "toc_Tred", "toc_Tform"):
                 stages[k] = stages.get(k, 0.0) + 1e3 * info[k] / steps
-            if dom.hps.leaf_event_pair is not None:
-                e0, e1 = dom.hps.leaf_event_pair
-                leaf_ms.append(e0.elapsed_time(e1))
-    return ts, tf, tsol, leaf_ms, sol, stages
+    return dict(t_step=sum(ts) / len(ts), t_factor=sum(tf) / len(tf), t_solve=sum(tsol) / len(tsol),
+                stages=stages, sol=sol, zero_pivots=info["zero_pivots"])
+
+
+def stage_table(dom, stages_ms, solve_ms, precision):
+    """Per-stage times against their algorithmic work and the peak of the
+    arithmetic they run in."""
+    W = algorithmic_work(dom.hps.p, dom.hps.n0, dom.hps.n1, dom.buf_pans, dom.Npan)
+    slab_prec = "fp32" if precision == "mixed" else "fp64"
+    rows = []
+
+    def add(name, ms, flops, prec):
+        tf = flops / (ms * 1e-3) / 1e12 if ms and ms > 0 else float("nan")
+        rows.append(dict(stage=name, ms=ms, algorithmic_gflop=flops / 1e9, achieved_tflops=tf,
+                         peak_tflops=PEAK_TFLOPS[prec], arithmetic=prec, frac=tf / PEAK_TFLOPS[prec]))
+
+    add("leaf DtN (batched LU + TRSM + GEMM, all leaves)", stages_ms["toc_DtN"], W["leaf"], slab_prec)
+    add("T_slab formation (multi-RHS rem sweeps)", stages_ms["toc_Tform"], W["tslab"], slab_prec)
+    add("interface block-tridiagonal factorization", stages_ms["toc_Tred"] - stages_ms["toc_Tform"],
+        W["interface"], "fp64")
+    rows.append(dict(stage="Aloc + Ared (elim LU, couplings, rem Schur + rem block LU)",
+                     ms=stages_ms["toc_Aloc"] + stages_ms["toc_Ared"]))
+    lf = dom.hps.leaf
+    fac_bytes = (lf["M"].numel() * lf["M"].element_size() if lf["M"] is not None else 0) + \
+        lf["DtN"].numel() * lf["DtN"].element_size() + dom.Aloc_solver.nbytes + dom.Ared_solver.nbytes + \
+        dom.Tred_solver.nbytes
+    rows.append(dict(stage="solve (all refinement steps)" if precision == "mixed" else "solve", ms=solve_ms,
+                     stored_factor_bytes=fac_bytes,
+                     note="HBM-bound: a solve reads the stored factors about once per sweep"))
+    return rows, W
+
+
+def time_rem_factor(dom, reps=5):
+    """Average duration (ms) of the twisted rem-chain factorization of this
+    build (all slabs; a chain launch and a middle-block launch of
+    k_btri_lu_factor, plus two block products between them), with HIP events
+    on the current stream, on fresh copies of the Schur blocks."""
+    import torch
+    from slablu_amd import hip
+    diag, sub, sup = dom.rem_schur_blocks()
+    b, m, bs, _ = diag.shape
+    ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device=diag.device)
+    ms = []
+    for _ in range(reps):
+        d, l, u = diag.clone(), sub.clone(), sup.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        hip.btri_factor(d, l, u, ipiv, twisted=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return sum(ms) / len(ms), (b, m, bs, str(diag.dtype).replace("torch.", ""))
+
+
+def measured_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def adjoint_leg(reps=2):
+    """Forward factor + solve and adjoint gradient times of the adjoint
+    workload (34 x 34 leaves, p = 16, 10 ppw, b = 1 + 0.1 * seeded Fourier
+    field, 1024 observation nodes, leaf factors recomputed in 4 chunks)."""
+    import torch
+    from slablu_amd.src_disc import built_in_funcs as bif
+    from slablu_amd.src_solver.adjoint import HelmholtzAdjoint, observed_loss
+    n0, p = 34, 16
+    n = n0 * (p - 2)
+    kh = (int(n / 10) + 0.03) * 2 * math.pi + 1.8
+    uu = lambda xx: bif.uu_dir_func_greens(xx, kh, center=SRC)
+    per_leaf = (p - 2) ** 2 * ((p - 2) ** 2 + 4 * (p - 2)) * 8
+    prob = HelmholtzAdjoint(torch.tensor([[0, 1.0], [0, 1.0]]), kh, 1 / (2 * n0), p, uu, buf_constant=1.0,
+                            leaf_memory_budget=n0 * n0 * per_leaf // 4)
+    b = bif.fourier_field(prob.xx).cuda()
+    obs = prob.observation_nodes(1024).cuda()
+    with torch.no_grad():
+        d = prob.solve(b + 0.02 * (bif.fourier_field(prob.xx, seed=1).cuda() - 1.0))[obs, 0]
+    tf, tb = [], []
+    for _ in range(reps + 1):
+        bt = b.clone().requires_grad_(True)
+        L = observed_loss(prob.solve(bt), obs, d)
+        L.backward()
+        tf.append(prob.timings["forward_s"])
+        tb.append(prob.timings["backward_s"])
+    tf, tb = tf[1:], tb[1:]
+    N = (p - 2) * (p * n0 * n0 + 2 * n0)
+    return dict(workload="Helmholtz -Lap u - kh^2 b u = 0, 34x34 leaves p=16 10ppw kh=%.2f, N=%d, b = 1 + 0.1 * "
+                         "(32 seeded Fourier modes |k|<=8), L = 1/2 ||u(obs) - d||^2 at 1024 interior nodes, leaf "
+                         "factors recomputed in 4 chunks" % (kh, N),
+                forward_ms=1e3 * sum(tf) / len(tf), gradient_ms=1e3 * sum(tb) / len(tb),
+                gradient_over_forward=sum(tb) / sum(tf))
 
 
 def cpu_baseline(n, p, ppw):
-    """The oracle (numpy/scipy FP64) on a bounded sample of the workload:
-    leaf DtNs of 100 leaves, the slab stage of one slab, two interface LU
-    steps and the interface solve, scaled to the full problem."""
+    """The oracle (numpy/scipy FP64) on a bounded sample of the headline
+    workload, extrapolated to the full problem: leaf DtNs of 100 leaves, the
+    slab stage of one slab (elim LU, rem Schur + LU, dense T_slab), two
+    interface LU steps of the interface size; solve phase: the leaf recovery
+    of 100 leaves (re-factored, as the reference's solve does), one slab's
+    two slab solves and the interface sweeps."""
     import numpy as np
     import scipy.linalg
-    from oracle import hps_oracle as O
-    cores = len(os.sched_getaffinity(0))
-    nwaves = int(n / ppw)
-    kh = (nwaves + 0.03) * 2 * math.pi + 1.8
-    a = 1 / (2 * (n / (p - 2)))
     import torch
+    from oracle import hps_oracle as O
     from slablu_amd.src_disc import built_in_funcs as bif
-    bfun = lambda xx, k: bif.bfield_lshaped_waveguide(torch.as_tensor(xx), k).numpy()
-    hps = O.HPSOracle(O.helmholtz_pdo(kh, bfun), np.array([[0, 1.0], [0, 1.0]]), a, p)
+    cores = len(os.sched_getaffinity(0))
+    kh = O.kh_from_ppw(n, ppw)
+    a = 1 / (2 * (n / (p - 2)))
+
+    def with_coef(h):
+        # the coefficient at the nodes from the field's cell-local evaluation
+        # (an input here, as the coefficient callables are to the solver)
+        h.c_nodes = bif.bfield_lshaped_waveguide(torch.as_tensor(h.xx_tot), kh).numpy().reshape(h.nboxes, -1)
+        return h
+
+    t0 = time.perf_counter()
+    hps = with_coef(O.HPSOracle(O.helmholtz_pdo(kh), np.array([[0, 1.0], [0, 1.0]]), a, p))
+    t_coef = time.perf_counter() - t0
     part = O.SlabPartition(hps.n0, hps.n1, p, 1.0)
     nleaf = min(100, hps.nboxes)
+    hps.dtns(np.arange(4))            # untimed: BLAS threads up
     t0 = time.perf_counter()
     hps.dtns(np.arange(nleaf))
     t_leaf = (time.perf_counter() - t0) * hps.nboxes / nleaf
-    # one slab of the two-level slab stage (elim LU, rem Schur + LU, dense T_slab)
+    # one slab of the two-level slab stage
     DtN = np.zeros((hps.nboxes, 4 * (p - 2), 4 * (p - 2)))
     boxes = np.arange(part.buf * hps.n1)
     DtN[boxes] = hps.dtns(boxes)
@@ -153,18 +231,38 @@ def cpu_baseline(n, p, ppw):
     L = hps.n1 * (p - 2)
     rng = np.random.default_rng(0)
     S = rng.standard_normal((L, L)) + L * np.eye(L)
+    C = rng.standard_normal((L, L))
     t0 = time.perf_counter()
     for _ in range(2):
         lu = scipy.linalg.lu_factor(S)
-        S2 = S - S @ scipy.linalg.lu_solve(lu, S)
+        S2 = S - C @ scipy.linalg.lu_solve(lu, C)
     t_tred = (time.perf_counter() - t0) / 2 * (part.Npan - 1)
-    t_total = t_leaf + t_slab + t_tred
+    # solve phase
+    nC = len(part.I_C)
+    f = rng.standard_normal((nC, 1))
+    t0 = time.perf_counter()
+    sl.slab_solve(0, f)
+    sl.slab_solve(0, f)
+    t_ssolve = (time.perf_counter() - t0) * part.Npan
+    t0 = time.perf_counter()
+    for _ in range(2 * (part.Npan - 1)):
+        scipy.linalg.lu_solve(lu, f[:L])
+    t_isolve = time.perf_counter() - t0
+    sub = with_coef(O.HPSOracle(O.helmholtz_pdo(kh), np.array([[0, 2 * a * 10], [0, 2 * a * 10]]), a, p))
+    t0 = time.perf_counter()
+    sub.leaf_solve(np.zeros((sub.xx_active.shape[0], 1)))
+    t_rec = (time.perf_counter() - t0) * hps.nboxes / sub.nboxes
+    t_factor = t_coef + t_leaf + t_slab + t_tred
+    t_solve = t_ssolve + t_isolve + t_rec
     N = O.dof_count(p, hps.n0, hps.n1)
-    return dict(value=N / t_total, unit="DOF/s", cores=cores, kind="port",
-                sample="oracle (numpy/scipy FP64, %d BLAS threads) on %d of %d leaves, 1 of %d slabs, 2 of %d "
-                       "interface LU steps; times scaled to the full %d-DOF problem (leaf %.1f s, slab %.1f s, "
-                       "interface %.1f s)" % (cores, nleaf, hps.nboxes, part.Npan, part.Npan - 1, N, t_leaf,
-                                             t_slab, t_tred))
+    return dict(value=N / (t_factor + t_solve), unit="DOF/s", cores=cores, kind="port",
+                sample="oracle (numpy/scipy FP64, %d BLAS threads), extrapolated factor+solve: leaf DtNs of %d of %d "
+                       "leaves, the slab stage of 1 of %d slabs, 2 of %d interface LU steps (random interface-size "
+                       "blocks), the leaf recovery of %d leaves (re-factored), one slab's two slab solves and the "
+                       "interface sweeps; scaled to the full %d-DOF problem (factor %.1f s: leaf %.1f, slab %.1f, "
+                       "interface %.1f; solve %.1f s)"
+                       % (cores, nleaf, hps.nboxes, part.Npan, part.Npan - 1, sub.nboxes, N, t_factor, t_leaf,
+                          t_slab, t_tred, t_solve))
 
 
 def main():
@@ -177,6 +275,11 @@ def main():
     ap.add_argument("--ppw", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline mixed path only (no constant-coefficient FP64 leg, no adjoint leg, no roofline "
+                         "re-run): one profiled step")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--domain", default="square", choices=["square", "sinusoidal", "annulus", "curvy_annulus"],
                     help="reference rectangle mapped to this domain (the c12 cross term for curved ones)")
     ap.add_argument("--box-x", type=float, default=1.0, help="width of the reference rectangle")
@@ -190,66 +293,91 @@ def main():
               "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
-    from oracle import hps_oracle as O
     torch.set_num_threads(len(os.sched_getaffinity(0)))
     n, p, ppw = args.n, args.p, args.ppw
-    print("bench: mixed path n=%d p=%d" % (n, p), file=sys.stderr, flush=True)
     extra = dict(periodic=args.periodic, domain=args.domain, box_x=args.box_x, pde=args.pde)
+    print("bench: mixed path n=%d p=%d" % (n, p), file=sys.stderr, flush=True)
     dom, uu_dir, kh = make_problem(n, p, ppw, "mixed", **extra)
-    N = O.dof_count(p, dom.hps.n0, dom.hps.n1)
-    ts, tf, tsol, leaf_ms, sol_mixed, stages = run_path(dom, uu_dir, args.steps, args.warmup, leaf_events=True)
-    refine_hist = list(dom.refine_history)
-    nboxes = dom.hps.nboxes
-    n0, n1 = dom.hps.n0, dom.hps.n1
-    roof_dom = time_leaf_solve(dom)
+    hps = dom.hps
+    N = (p - 2) * (p * hps.n0 * hps.n1 + hps.n0 + hps.n1)
+    mx = run_path(dom, uu_dir, args.steps, args.warmup)
+    refine = dict(converged=dom.refine_converged, corrections=list(dom.refine_history),
+                  residuals=list(dom.refine_residuals))
+    stages, W = stage_table(dom, mx["stages"], 1e3 * mx["t_solve"], "mixed")
+    roof = None
+    if not args.no_extras:
+        print("bench: rem factorization timing", file=sys.stderr, flush=True)
+        avg_ms, shape = time_rem_factor(dom)
+        nb_, m_, bs_, dt_ = shape
+        ach = W["rem_factor"] / (avg_ms * 1e-3) / 1e12
+        roof = dict(bound="mfma", kernel="k_btri_lu_factor<float,%d>: twisted block LU of the rem chains, %d slabs x "
+                                         "%d blocks of %d x %d (%s); 2 launches per factorization (chain, middle)"
+                                         % ((bs_ + 15) // 16, nb_, m_, bs_, bs_, dt_),
+                    achieved=ach, peak=PEAK_TFLOPS["fp32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["fp32"],
+                    traffic=measured_traffic("rem_factor_traffic.json"),
+                    algorithmic_flops_per_launch=W["rem_factor"] / 2, avg_ms_per_launch=avg_ms / 2,
+                    factorization_ms=avg_ms)
     del dom
     torch.cuda.empty_cache()
     fp64 = None
+    accuracy_ok = bool(refine["converged"])
     if not args.no_fp64:
         print("bench: fp64 path", file=sys.stderr, flush=True)
         dom64, uu_dir, _ = make_problem(n, p, ppw, "fp64", **extra)
-        ts64, tf64, tsol64, leaf64, sol64, stages64 = run_path(dom64, uu_dir, max(1, args.steps), args.warmup,
-                                                                leaf_events=True)
-        err = (torch.linalg.norm(sol_mixed - sol64) / torch.linalg.norm(sol64)).item()
-        fp64 = dict(dof_per_s=N / (sum(ts64) / len(ts64)), ms_per_step=1e3 * sum(ts64) / len(ts64),
-                    factor_ms=1e3 * sum(tf64) / len(tf64), solve_ms=1e3 * sum(tsol64) / len(tsol64),
-                    leaf_ms=sum(leaf64) / max(1, len(leaf64)), stages_ms=stages64, err_mixed_vs_fp64=err)
+        r64 = run_path(dom64, uu_dir, max(1, min(args.steps, 3)), min(args.warmup, 1))
+        err = (torch.linalg.norm(mx["sol"] - r64["sol"]) / torch.linalg.norm(r64["sol"])).item()
+        st64, _ = stage_table(dom64, r64["stages"], 1e3 * r64["t_solve"], "fp64")
+        fp64 = dict(dof_per_s=N / r64["t_step"], ms_per_step=1e3 * r64["t_step"], factor_ms=1e3 * r64["t_factor"],
+                    solve_ms=1e3 * r64["t_solve"], stages=st64, err_mixed_vs_fp64=err)
+        accuracy_ok = accuracy_ok and err <= 1e-8
         del dom64
         torch.cuda.empty_cache()
-    t_step = sum(ts) / len(ts)
-    roof = None
-    if roof_dom is not None:
-        avg_ms, nb, ni, isz = roof_dom
-        nbytes = leaf_solve_bytes(nb, ni, isz)
-        ach = nbytes / (avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", kernel="k_getrs_wave<float,1>: leaf interior solve A_cc^{-1} r over %d leaves "
-                                          "(%d x %d FP32 LU factors), run in every refinement step" % (nb, ni, ni),
-                    achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                    traffic=measured_traffic(), algorithmic_bytes=nbytes, avg_ms=avg_ms)
+    cfg2 = adj = None
+    if not args.no_extras and not args.no_fp64:
+        print("bench: constant-coefficient fp64", file=sys.stderr, flush=True)
+        dom2, uu2, _ = make_problem(n, p, ppw, "fp64", pde="constant")
+        r2 = run_path(dom2, uu2, max(1, min(args.steps, 3)), min(args.warmup, 1))
+        cfg2 = dict(workload="constant-coefficient Helmholtz, n=%d p=%d ppw=%d, FP64" % (n, p, ppw),
+                    dof_per_s=N / r2["t_step"], ms_per_step=1e3 * r2["t_step"], factor_ms=1e3 * r2["t_factor"],
+                    solve_ms=1e3 * r2["t_solve"])
+        del dom2
+        torch.cuda.empty_cache()
+        print("bench: adjoint", file=sys.stderr, flush=True)
+        adj = adjoint_leg()
+    if args.calibrate_copy:
+        src = torch.ones(64 * 2 ** 20, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(n, p, ppw)
+    t_step = mx["t_step"]
     out = {
         "metric": METRIC, "value": N / t_step, "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": "Helmholtz, %s, %s, n=%d p=%d ppw=%d kh=%.2f, "
-                               "%dx%d leaves, N=%d, mixed precision + FP64 refinement%s"
+        "config": {"workload": "Helmholtz, %s, %s, n=%d p=%d ppw=%d kh=%.2f, %dx%d leaves, N=%d, source at (1.5, "
+                               "1.5), mixed precision + FP64 refinement to 1e-8%s"
                                % ("L-shaped waveguide bumps" if args.pde == "waveguide" else "constant coefficient",
                                   "unit square" if args.domain == "square" and args.box_x == 1.0
                                   else "%s domain (reference box %g x 1)" % (args.domain, args.box_x),
-                                  n, p, ppw, kh, n0, n1, N,
-                                  ", x-periodic" if args.periodic else ""),
-                   "N": N, "p": p, "leaves": nboxes, "precision": "mixed", "periodic": bool(args.periodic)},
-        "factor_ms": 1e3 * sum(tf) / len(tf), "solve_ms": 1e3 * sum(tsol) / len(tsol),
-        "refine_corrections": refine_hist, "stages_ms": stages,
-        "fp64": fp64,
-        "speedup_vs_fp64": (fp64["ms_per_step"] / (1e3 * t_step)) if fp64 else None,
+                                  n, p, ppw, kh, hps.n0, hps.n1, N, ", x-periodic" if args.periodic else ""),
+                   "N": N, "p": p, "leaves": hps.nboxes, "precision": "mixed", "periodic": bool(args.periodic)},
+        "factor_ms": 1e3 * mx["t_factor"], "solve_ms": 1e3 * mx["t_solve"],
+        "refinement": refine, "accuracy_ok": accuracy_ok, "stages": stages,
+        "fp64": fp64, "speedup_vs_fp64": (fp64["ms_per_step"] / (1e3 * t_step)) if fp64 else None,
+        "cfg2_fp64": cfg2, "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    sys.stdout.flush()
+    if not accuracy_ok:
+        print("bench: the mixed solution missed the 1e-8 target against FP64 (or the refinement did not converge)",
+              file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

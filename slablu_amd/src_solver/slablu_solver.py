@@ -370,7 +370,20 @@ class Domain_Solver:
 
     def construct_Ared(self):
         """Schur complement of each slab onto its interior horizontal faces,
-        block tridiagonal over the n1-1 face rows (blocks buf*f)."""
+        block tridiagonal over the n1-1 face rows (blocks buf*f), factored."""
+        diag, sub, sup = self.rem_schur_blocks()
+        # FP32 (mixed precision): triangular inverses of the diagonal blocks,
+        # so the T_slab and refinement sweeps are block products; the FP64
+        # path keeps the substitutions
+        self.Ared_solver = Block_Tridiagonal(self.Npan, self.Npan_loc - 1, self.rem_bs, twisted=True,
+                                             inv_factors=self.precision == "mixed")
+        self.Ared_solver.build_sweep(diag, sub, sup)
+
+    def rem_schur_blocks(self):
+        """(diag (Npan, n1-1, B, B), sub, sup (Npan, n1-2, B, B)) of the rem
+        Schur complements A_rr - sum_j A_r,e_j A_e_j^{-1} A_e_j,r, before
+        factorization (slablu_solver.py:768-882); also keeps the coupling
+        solves W, Y and C Y the later stages use."""
         Np, n1, buf, f = self.Npan, self.Npan_loc, self.buf_pans, self.hps.p - 2
         m, B = buf - 1, buf * f
         self._couplings()
@@ -389,11 +402,7 @@ class Domain_Solver:
         diag = Arr_d - CW[:, :mr, 1, :, 1, :] - CW[:, 1:, 0, :, 0, :]
         sup = Arr_u - CW[:, 1:mr, 0, :, 1, :]
         sub = Arr_l - CW[:, 1:mr, 1, :, 0, :]
-        # FP32 (mixed precision): triangular inverses of the diagonal blocks,
-        # so the T_slab and refinement sweeps are block products; the FP64
-        # path keeps the substitutions
-        self.Ared_solver = Block_Tridiagonal(Np, mr, B, twisted=True, inv_factors=self.precision == "mixed")
-        self.Ared_solver.build_sweep(diag.contiguous(), sub.contiguous(), sup.contiguous())
+        return diag.contiguous(), sub.contiguous(), sup.contiguous()
 
     def _slab_u_local(self, upad, s):
         """(n1, 2B, ncols) view of the padded rem solution of slab s: rows of
