@@ -128,21 +128,23 @@ def time_rem_factor(dom, reps=5):
     build (all slabs; a chain launch and a middle-block launch of
     k_btri_lu_factor, plus two block products between them), with HIP events
     on the current stream, on fresh copies of the Schur blocks."""
+    import copy
     import torch
-    from slablu_amd import hip
     diag, sub, sup = dom.rem_schur_blocks()
     b, m, bs, _ = diag.shape
-    ipiv = torch.zeros(b, m, bs, dtype=torch.int32, device=diag.device)
+    proto = dom.Ared_solver
     ms = []
     for _ in range(reps):
         d, l, u = diag.clone(), sub.clone(), sup.clone()
+        S = copy.copy(proto)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        hip.btri_factor(d, l, u, ipiv, twisted=True)
+        S.build_sweep(d, l, u)
         e1.record()
         torch.cuda.synchronize()
         ms.append(e0.elapsed_time(e1))
+        del S
     return sum(ms) / len(ms), (b, m, bs, str(diag.dtype).replace("torch.", ""))
 
 

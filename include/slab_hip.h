@@ -251,6 +251,32 @@ int slab_leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc
                        const double* nx_val, double* rI, int64_t rI_ld, int64_t rI_bs, double* fl, int64_t fl_ld,
                        int64_t fl_bs, double* uu, int64_t uu_ld, int64_t uu_bs, void* stream);
 
+/* Right-hand-side tile width (nr in {1, 2, 4, 8, 16, 32}; 0 restores the
+ * built-in rule) of the one-workgroup-per-chain sweeps for blocks of size
+ * bs in dtype, LU form (inv_factors = 0) or inverse-factor form (1): the
+ * knob the autotuner (slablu_amd/autotune.py) sets after timing candidate
+ * widths on the actual chains.  Process-wide; returns the previous width,
+ * -1 for bad arguments.  No counterpart in the reference (its sweeps are
+ * torch library calls, block_tridiag_solver_dense.py:179-222). */
+int slab_set_sweep_tile(int dtype, int bs, int inv_factors, int nr);
+
+/* Batched Krylov products of the FP64 refinement, all right-hand-side
+ * columns in one launch: out[c*out_cs + i] = V_c[i] . w_c for i < k <= 32,
+ * where vector i of column c starts at V + c*v_cs + i*v_ks and w_c at
+ * w + c*w_cs (n entries each); ws: slab_mdot_ws(nc, k, n) bytes of device
+ * workspace for the fixed-order partial sums.  No counterpart in the
+ * reference (its solve is direct); used by the FGMRES refinement of the
+ * mixed-precision form (Gram-Schmidt projections, norms, Gram matrices). */
+size_t slab_mdot_ws(int nc, int k, int64_t n);
+int slab_mdot(int nc, int k, int64_t n, const double* V, int64_t v_cs, int64_t v_ks, const double* w, int64_t w_cs,
+              double* out, int64_t out_cs, void* ws, void* stream);
+
+/* w_c += alpha * sum_{i<k} coef[c*coef_cs + i] V_c[i] for every column c
+ * (layouts as slab_mdot); the Gram-Schmidt update and the FGMRES solution
+ * update of the refinement. */
+int slab_maxpy(int nc, int k, int64_t n, double alpha, const double* V, int64_t v_cs, int64_t v_ks,
+               const double* coef, int64_t coef_cs, double* w, int64_t w_cs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

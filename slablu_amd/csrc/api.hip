@@ -364,6 +364,27 @@ int slab_leaf_assemble_nodes(int dtype, int nboxes, int p, int nxc, const int* x
   return -1;
 }
 
+int slab_set_sweep_tile(int dtype, int bs, int inv_factors, int nr) {
+  return set_sweep_tile(dtype, bs, inv_factors, nr);
+}
+
+size_t slab_mdot_ws(int nc, int k, int64_t n) {
+  if (nc < 0 || k < 0 || n < 0) return 0;
+  return mdot_ws_elems(nc, k, n) * sizeof(double);
+}
+
+int slab_mdot(int nc, int k, int64_t n, const double* V, int64_t v_cs, int64_t v_ks, const double* w, int64_t w_cs,
+              double* out, int64_t out_cs, void* ws, void* stream) {
+  if (nc < 0 || k < 0 || k > 32 || n < 0 || (nc > 0 && k > 0 && (!V || !w || !out || !ws))) return -1;
+  return status(mdot(nc, k, n, V, v_cs, v_ks, w, w_cs, out, out_cs, reinterpret_cast<double*>(ws), S(stream)));
+}
+
+int slab_maxpy(int nc, int k, int64_t n, double alpha, const double* V, int64_t v_cs, int64_t v_ks,
+               const double* coef, int64_t coef_cs, double* w, int64_t w_cs, void* stream) {
+  if (nc < 0 || k < 0 || k > 32 || n < 0 || (nc > 0 && k > 0 && n > 0 && (!V || !coef || !w))) return -1;
+  return status(maxpy(nc, k, n, alpha, V, v_cs, v_ks, coef, coef_cs, w, w_cs, S(stream)));
+}
+
 int slab_leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
                        const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
                        const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,

@@ -2,7 +2,28 @@
 // lusolve_sweep of src_solver/block_tridiag_solver_dense.py:179-203).
 #include "lds_dev.h"
 
+#include <atomic>
+
 namespace slab {
+
+// Right-hand-side tile widths chosen by the autotuner, per (dtype, block
+// size, inverse factors): 0 = the built-in rule.  Process-wide; read at every
+// sweep launch.
+namespace {
+constexpr int kTileBsMax = 256;
+std::atomic<int> g_sweep_tile[2][kTileBsMax + 1][2];
+}  // namespace
+
+int sweep_tile_override(int dtype, int bs, bool inv) {
+  if (dtype < 0 || dtype > 1 || bs < 1 || bs > kTileBsMax) return 0;
+  return g_sweep_tile[dtype][bs][inv ? 1 : 0].load(std::memory_order_relaxed);
+}
+
+int set_sweep_tile(int dtype, int bs, int inv, int nr) {
+  if (dtype < 0 || dtype > 1 || bs < 1 || bs > kTileBsMax) return -1;
+  if (nr != 0 && nr != 1 && nr != 2 && nr != 4 && nr != 8 && nr != 16 && nr != 32) return -1;
+  return g_sweep_tile[dtype][bs][inv ? 1 : 0].exchange(nr);
+}
 
 namespace {
 
@@ -487,6 +508,13 @@ hipError_t solve_sets(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, int6
   // small blocks in substitution form: 8-column tiles (about 130 VGPRs, three
   // workgroups per CU) beat 32-column ones (one workgroup per CU)
   if (bs <= 32 && c0.linv == nullptr && nr > 8) nr = 8;
+  // a tile width chosen by the autotuner for this block size, if any (never
+  // wider than the columns there are)
+  const int tuned = sweep_tile_override(sizeof(T) == 8 ? 0 : 1, bs, c0.linv != nullptr);
+  if (tuned > 0 && nrhs > 1) {
+    nr = tuned;
+    while (nr > 1 && nr > nrhs) nr >>= 1;
+  }
   while (nr > 1 && lu_solve_lds<T>(bs, nr) > kLdsMax) nr >>= 1;
   if (lu_solve_lds<T>(bs, nr) > kLdsMax || bs > (sizeof(T) == 8 ? 112 : 160)) return hipErrorInvalidValue;
   switch (nr) {

@@ -55,6 +55,14 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
 }  // namespace slab
 
 namespace slab {
+// Batched Krylov products (FP64): out[c*ocs + i] = V_c[i] . w_c for i < k
+// (V_c[i] at V + c*vcs + i*vks, n entries each; ws holds mdot_ws_elems
+// partial sums), and w_c += alpha sum_i coef[c*ccs + i] V_c[i].
+size_t mdot_ws_elems(int nc, int k, int64_t n);
+hipError_t mdot(int nc, int k, int64_t n, const double* V, int64_t vcs, int64_t vks, const double* w, int64_t wcs,
+                double* out, int64_t ocs, double* ws, hipStream_t s);
+hipError_t maxpy(int nc, int k, int64_t n, double alpha, const double* V, int64_t vcs, int64_t vks,
+                 const double* coef, int64_t ccs, double* w, int64_t wcs, hipStream_t s);
 hipError_t leaf_residual(int nboxes, int p, int nrhs, const double* uc, int64_t uc_ld, int64_t uc_bs,
                          const double* ub, int64_t ub_ld, int64_t ub_bs, const double* Intmap, const int* Jc,
                          const int* Jxreo, const double* D, const double* Dsq, const double* c11, const double* c22,
@@ -96,6 +104,11 @@ hipError_t piv_to_perm_batched(int batch, int n, const int* ipiv, int64_t pbs, i
 template <typename T>
 hipError_t getrs_small(int batch, int n, int nrhs, View<T> LU, const int* ipiv, int64_t pbs, View<T> B,
                        hipStream_t s);
+// autotuned right-hand-side tile width of the chain sweeps for (dtype 0 =
+// FP64 / 1 = FP32, block size, inverse factors); 0 = built-in rule.  set
+// returns the previous width or -1 for bad arguments.
+int sweep_tile_override(int dtype, int bs, bool inv);
+int set_sweep_tile(int dtype, int bs, int inv, int nr);
 // LU-form block-tridiagonal factorization / sweeps, one workgroup per chain
 template <typename T>
 bool btri_lu_fused_ok(int bs);

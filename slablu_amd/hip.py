@@ -61,6 +61,10 @@ def _load():
                                 _vp], _int),
         "slab_leaf_assemble_nodes": ([_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _i64, _i64, _vp], _int),
+        "slab_set_sweep_tile": ([_int, _int, _int, _int], _int),
+        "slab_mdot_ws": ([_int, _int, _i64], ctypes.c_size_t),
+        "slab_mdot": ([_int, _int, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp], _int),
+        "slab_maxpy": ([_int, _int, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp], _int),
         "slab_leaf_residual": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp],
                                _int),
@@ -342,6 +346,41 @@ def btri_inv_solve(diag, sub, sup, f):
     _check(LIB.slab_btri_inv_solve(_dt(diag), b, m, bs, _p(diag), ds, di, _p(sub), ls, li, _p(sup), us, ui, _p(f),
                                    f.stride(0), f.stride(1), nrhs, _p(ws), _stream()), "slab_btri_inv_solve")
     return f
+
+
+def set_sweep_tile(dtype, bs, inv_factors, nr):
+    """Right-hand-side tile width of the chain sweeps for blocks of size bs
+    (0: built-in rule); returns the previous width."""
+    rc = LIB.slab_set_sweep_tile(F64 if dtype == torch.float64 else F32, int(bs), int(bool(inv_factors)), int(nr))
+    if rc < 0:
+        raise ValueError("slab_set_sweep_tile(%s, %d, %d, %d) rejected" % (dtype, bs, inv_factors, nr))
+    return rc
+
+
+def mdot(V, w, out=None):
+    """out[c, i] = V[c, i, :] . w[c, :] for all columns c at once (FP64):
+    V (nc, k, n) and w (nc, n) with unit stride along n; out (nc, k)."""
+    nc, k, n = V.shape
+    _require(V.dtype == torch.float64 and w.dtype == torch.float64 and V.stride(2) == 1 and w.stride(-1) == 1 and
+             tuple(w.shape) == (nc, n) and 1 <= k <= 32, "mdot layouts")
+    if out is None:
+        out = torch.empty(nc, k, dtype=torch.float64, device=V.device)
+    _require(out.stride(-1) == 1 and tuple(out.shape) == (nc, k), "mdot output layout")
+    ws = torch.empty(max(1, int(LIB.slab_mdot_ws(nc, k, n)) // 8), dtype=torch.float64, device=V.device)
+    _check(LIB.slab_mdot(nc, k, n, _p(V), V.stride(0), V.stride(1), _p(w), w.stride(0), _p(out), out.stride(0),
+                         _p(ws), _stream()), "slab_mdot")
+    return out
+
+
+def maxpy(V, coef, w, alpha=1.0):
+    """w[c, :] += alpha sum_i coef[c, i] V[c, i, :] for all columns c (FP64)."""
+    nc, k, n = V.shape
+    _require(V.dtype == torch.float64 and w.dtype == torch.float64 and coef.dtype == torch.float64 and
+             V.stride(2) == 1 and w.stride(-1) == 1 and coef.stride(-1) == 1 and tuple(w.shape) == (nc, n) and
+             tuple(coef.shape) == (nc, k) and 1 <= k <= 32, "maxpy layouts")
+    _check(LIB.slab_maxpy(nc, k, n, float(alpha), _p(V), V.stride(0), V.stride(1), _p(coef), coef.stride(0), _p(w),
+                          w.stride(0), _stream()), "slab_maxpy")
+    return w
 
 
 def leaf_assemble(M, p, border, D, Dsq, coefs, Nxtot):

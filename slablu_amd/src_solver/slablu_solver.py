@@ -42,6 +42,7 @@ import torch
 
 from .. import hip
 from ..src_disc import hps_multidomain_disc
+from .block_tridiag_separated import SeparatedBlockTridiagonal, separated_period
 from .block_tridiag_solver_dense import Block_Tridiagonal
 from .block_tridiag_solver_structured import HBS_BlockTridiagonal
 
@@ -104,6 +105,9 @@ class Domain_Solver:
         # are needed (slablu_solver.py:781-790 sizes its chunks from free
         # memory); leaf_chunk forces a chunk size
         self.leaf_memory_budget, self.leaf_chunk = leaf_memory_budget, leaf_chunk
+        # rem chains eliminated piecewise around separator blocks when long
+        # enough (SeparatedBlockTridiagonal); False: one twisted chain each
+        self.separated_rem = True
         # stage timings of build() from events on the stream (no host
         # synchronization between stages)
         self.stage_sync = True
@@ -372,11 +376,18 @@ class Domain_Solver:
         """Schur complement of each slab onto its interior horizontal faces,
         block tridiagonal over the n1-1 face rows (blocks buf*f), factored."""
         diag, sub, sup = self.rem_schur_blocks()
+        m, B = self.Npan_loc - 1, self.rem_bs
         # FP32 (mixed precision): triangular inverses of the diagonal blocks,
         # so the T_slab and refinement sweeps are block products; the FP64
         # path keeps the substitutions
-        self.Ared_solver = Block_Tridiagonal(self.Npan, self.Npan_loc - 1, self.rem_bs, twisted=True,
-                                             inv_factors=self.precision == "mixed")
+        inv = self.precision == "mixed"
+        d = separated_period(m, B, self.wdt) if self.separated_rem else None
+        if d is not None:
+            # pieces of d-1 blocks between separator blocks: far fewer
+            # dependent block steps than one chain (block_tridiag_separated)
+            self.Ared_solver = SeparatedBlockTridiagonal(self.Npan, m, B, d, inv_factors=inv)
+        else:
+            self.Ared_solver = Block_Tridiagonal(self.Npan, m, B, twisted=True, inv_factors=inv)
         self.Ared_solver.build_sweep(diag, sub, sup)
 
     def rem_schur_blocks(self):
@@ -895,18 +906,17 @@ class Domain_Solver:
                 pack(wI, wF, w)
                 h = torch.zeros(nc, j + 2, dtype=torch.float64, device=dev)
                 for _ in range(2):                                  # classical Gram-Schmidt, twice
-                    for col in range(nc):                           # matrix-vector products on the basis
-                        Vb = V[col, :j + 1]                         # (j+1, ntot), rows contiguous
-                        c = torch.mv(Vb, w[col])
-                        w[col].addmv_(Vb.t(), c, alpha=-1.0)
-                        h[col, :j + 1] += c
-                h[:, j + 1] = torch.linalg.norm(w, dim=1)
+                    # projections of every column's w on its basis, one
+                    # batched launch each (slab_mdot / slab_maxpy)
+                    c = hip.mdot(V[:, :j + 1], w)
+                    hip.maxpy(V[:, :j + 1], c, w, alpha=-1.0)
+                    h[:, :j + 1] += c
+                h[:, j + 1] = hip.mdot(w[:, None, :], w)[:, 0].sqrt()
                 V[:, j + 1] = w / h[:, j + 1].clamp_min(1e-300)[:, None]
                 gr = torch.empty(nc, j + 2, dtype=torch.float64, device=dev)
-                for col in range(nc):
-                    zf = Z[col, j, nI:]
-                    torch.mv(Z[col, :j + 1, nI:], zf, out=gr[col, :j + 1])
-                    gr[col, j + 1] = torch.dot(zf, xc0[col])
+                zf = Z[:, j, nI:]
+                hip.mdot(Z[:, :j + 1, nI:], zf, out=gr[:, :j + 1])
+                gr[:, j + 1] = hip.mdot(xc0[:, None, :], zf)[:, 0]
                 hg = torch.cat((h, gr), 1).cpu().numpy()
                 H[:, :j + 2, j] = hg[:, :j + 2]
                 G[:, :j + 1, j] = hg[:, j + 2:2 * j + 3]
@@ -940,10 +950,10 @@ class Domain_Solver:
                 prev = rel if steps >= 2 else None
                 if done or steps >= self.refine_maxit:
                     break
-            yd = torch.from_numpy(y_prev).to(dev)
+            yd = torch.from_numpy(np.ascontiguousarray(y_prev)).to(dev)
             upd = torch.zeros(nc, ntot, dtype=torch.float64, device=dev)
-            for i in range(y_prev.shape[1]):
-                upd.addcmul_(Z[:, i], yd[:, i:i + 1])
+            if y_prev.shape[1] > 0:
+                hip.maxpy(Z[:, :y_prev.shape[1]], yd, upd)
             dI, dC = unpack(upd)
             x_face[Ic] += dC
             uc += dI
