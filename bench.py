@@ -18,13 +18,19 @@ Beside it, in the same JSON line:
   cfg2_fp64     the constant-coefficient problem, FP64, same size;
   adjoint       34 x 34 leaves, p=16 (N = 259,896): forward (factor + solve)
                 and gradient (adjoint) times of the per-node coefficient;
+  multi_rhs     132 x 132 leaves, p=16 (N = 3,906,672), 64 right-hand sides:
+                FP64 factor + graph-replayed 64-column solve (autotuned sweep
+                tiles), and the mixed + refined form, error against FP64;
   stages        per-stage times with their algorithmic work (fixed formulas)
                 and the fraction of the MI355X peak they reach;
-  roofline      the dominant kernel of the mixed step, the rem-chain block LU
-                (k_btri_lu_factor<float, RT>, twisted: a chain launch and a
-                middle-block launch per factorization), timed with HIP events
-                on its stream over a re-run of the rem factorization of this
-                build; traffic from profiles/rem_factor_traffic.json when present;
+  rem_factorization  the rem-chain factorization re-run on this build's
+                Schur blocks (HIP events), against its algorithmic flops;
+  roofline      the dominant kernel of the mixed step, the FP32 MFMA GEMM
+                k_gemm<float, int, true, true>: every launch of it in the
+                timed steps bracketed by HIP events on its stream (the
+                library's launch probe), algorithmic flops 2 m n k per product
+                summed over the launches; traffic (HBM bytes per launch) from
+                profiles/gemm_f32_traffic.json when present;
   cpu_baseline  the oracle (numpy/scipy FP64) on a bounded sample of the same
                 workload, factor and solve phases, extrapolated.
 """
@@ -73,12 +79,19 @@ def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.
     return dom, uu_dir, kh
 
 
-def run_path(dom, uu_dir, steps, warmup):
+def run_path(dom, uu_dir, steps, warmup, probe=None):
+    """warmup + steps factor+solve steps; with probe (a hip.PROBE_* kernel
+    id) the launch probe times that kernel over the timed steps."""
     import torch
+    from slablu_amd import hip
     ts, tf, tsol = [], [], []
     stages = {}
     sol = None
+    probed = None
     for it in range(warmup + steps):
+        if probe is not None and it == warmup:
+            torch.cuda.synchronize()
+            hip.probe_start(probe, 1 << 16)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         info = dom.build(verbose=False)
@@ -91,8 +104,11 @@ def run_path(dom, uu_dir, steps, warmup):
             tsol.append(toc)
             for k in ("toc_DtN", "toc_Aloc", "toc_Ared", "toc_Tred", "toc_Tform"):
                 stages[k] = stages.get(k, 0.0) + 1e3 * info[k] / steps
+    if probe is not None:
+        probed = hip.probe_stop()
+        probed["steps"] = steps
     return dict(t_step=sum(ts) / len(ts), t_factor=sum(tf) / len(tf), t_solve=sum(tsol) / len(tsol),
-                stages=stages, sol=sol, zero_pivots=info["zero_pivots"])
+                stages=stages, sol=sol, zero_pivots=info["zero_pivots"], probe=probed)
 
 
 def stage_table(dom, stages_ms, solve_ms, precision):
@@ -124,10 +140,10 @@ def stage_table(dom, stages_ms, solve_ms, precision):
 
 
 def time_rem_factor(dom, reps=5):
-    """Average duration (ms) of the twisted rem-chain factorization of this
-    build (all slabs; a chain launch and a middle-block launch of
-    k_btri_lu_factor, plus two block products between them), with HIP events
-    on the current stream, on fresh copies of the Schur blocks."""
+    """Average duration (ms) of the rem-chain factorization of this build
+    (all slabs: pieces, spikes and separators of the separated chains, or
+    the twisted single chains), with HIP events on the current stream, on
+    fresh copies of the Schur blocks."""
     import copy
     import torch
     diag, sub, sup = dom.rem_schur_blocks()
@@ -188,6 +204,72 @@ def adjoint_leg(reps=2):
                          "factors recomputed in 4 chunks" % (kh, N),
                 forward_ms=1e3 * sum(tf) / len(tf), gradient_ms=1e3 * sum(tb) / len(tb),
                 gradient_over_forward=sum(tb) / sum(tf))
+
+
+def multirhs_leg(reps=2):
+    """The batched multi-right-hand-side workload: 132 x 132 leaves, p = 16
+    (N = 3,906,672), waveguide coefficient, 10 ppw, 64 Green's-function
+    columns from sources uniform in the annulus 1.2 < |x - (0.5, 0.5)| < 2
+    (seed 0).  FP64: factor, sweep tiles autotuned, the 64-column solve
+    captured as one HIP graph and replayed; mixed: factor and the refined
+    64-column solve.  Times are means over reps after a warm-up."""
+    import torch
+    from slablu_amd import autotune
+    from slablu_amd.src_disc import pdo, built_in_funcs as bif
+    from slablu_amd.src_solver.slablu_solver import Domain_Solver
+    n0, p, nc = 132, 16, 64
+    n = n0 * (p - 2)
+    kh = (int(n / 10) + 0.03) * 2 * math.pi + 1.8
+    op = pdo.PDO_2d(pdo.ones, pdo.ones, c=lambda xx: bif.bfield_lshaped_waveguide(xx, kh))
+    srcs = bif.annulus_sources(nc, seed=0)
+    uu = lambda xx: bif.uu_dir_func_greens_multi(xx, kh, srcs)
+    N = (p - 2) * (p * n0 * n0 + 2 * n0)
+    out = dict(workload="Helmholtz, L-shaped waveguide bumps, 132x132 leaves p=16 10ppw kh=%.2f, N=%d, %d "
+                        "right-hand sides (annulus sources, seed 0)" % (kh, N, nc))
+    ref = None
+    for precision in ("fp64", "mixed"):
+        dom = Domain_Solver(torch.tensor([[0, 1.0], [0, 1.0]]), op, kh, 1 / (2 * n0), p=p, buf_constant=1.0,
+                            precision=precision)
+        dom.build(verbose=False)
+        tuning = autotune.tune_sweeps(dom, nc)
+        tfac, tsol = [], []
+        for it in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dom.build(verbose=False)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if precision == "fp64":
+                dom.solve_graph(uu)       # capture (after two eager warm-ups) once per build
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+                sol = dom.solve_graph(uu)
+            else:
+                ts = t1
+                sol = dom.solve(uu)[0]
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if it > 0:
+                tfac.append(t1 - t0)
+                tsol.append(t2 - ts)
+        fac_ms, sol_ms = 1e3 * sum(tfac) / reps, 1e3 * sum(tsol) / reps
+        leg = dict(factor_ms=fac_ms, solve_ms=sol_ms, dof_rhs_per_s=N * nc / ((fac_ms + sol_ms) * 1e-3),
+                   tiles={k: v["nr"] for k, v in tuning.items()},
+                   rem_chains="%s, %d x %d blocks of %d" % (type(dom.Ared_solver).__name__, dom.Npan, dom.rem_nblocks,
+                                                          dom.rem_bs))
+        if precision == "fp64":
+            leg["solve"] = "64 columns, one captured HIP graph, replayed"
+            ref = sol.cpu()
+        else:
+            leg["solve"] = "64 columns, FP32 two-level preconditioner, FP64 FGMRES refinement to 1e-8"
+            leg["refinement_converged"] = bool(dom.refine_converged)
+            leg["corrections"] = len(dom.refine_history)
+            err = torch.linalg.norm(sol.cpu() - ref, dim=0) / torch.linalg.norm(ref, dim=0)
+            leg["max_column_err_vs_fp64"] = err.max().item()
+        out[precision] = leg
+        del dom, sol
+        torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(n, p, ppw):
@@ -302,23 +384,35 @@ def main():
     dom, uu_dir, kh = make_problem(n, p, ppw, "mixed", **extra)
     hps = dom.hps
     N = (p - 2) * (p * hps.n0 * hps.n1 + hps.n0 + hps.n1)
-    mx = run_path(dom, uu_dir, args.steps, args.warmup)
+    from slablu_amd import hip
+    mx = run_path(dom, uu_dir, args.steps, args.warmup, probe=hip.PROBE_GEMM_F32)
     refine = dict(converged=dom.refine_converged, corrections=list(dom.refine_history),
                   residuals=list(dom.refine_residuals))
     stages, W = stage_table(dom, mx["stages"], 1e3 * mx["t_solve"], "mixed")
     roof = None
+    pr = mx["probe"]
+    if pr and pr["launches"] > 0:
+        nl = pr["launches"]
+        avg_ms = pr["total_ms"] / nl
+        fl, by = pr["flops"] / nl, pr["bytes"] / nl
+        ach = fl / (avg_ms * 1e-3) / 1e12
+        roof = dict(bound="mfma",
+                    kernel="k_gemm<float, int, true, true>: FP32 MFMA GEMM (64 x 64 output tiles, 16-byte rows of "
+                           "A, B and C), every launch in the timed steps (leaf LU trailing updates and DtN "
+                           "products, slab Schur products)",
+                    achieved=ach, peak=PEAK_TFLOPS["fp32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["fp32"],
+                    traffic=measured_traffic("gemm_f32_traffic.json"),
+                    algorithmic_flops_per_launch=fl, algorithmic_bytes_per_launch=by, avg_ms_per_launch=avg_ms,
+                    launches_per_step=nl / pr["steps"], untimed_launches=pr["dropped"],
+                    timing="HIP events around each launch on its stream (the library's launch probe), timed steps")
+    remf = None
     if not args.no_extras:
         print("bench: rem factorization timing", file=sys.stderr, flush=True)
         avg_ms, shape = time_rem_factor(dom)
         nb_, m_, bs_, dt_ = shape
-        ach = W["rem_factor"] / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", kernel="k_btri_lu_factor<float,%d>: twisted block LU of the rem chains, %d slabs x "
-                                         "%d blocks of %d x %d (%s); 2 launches per factorization (chain, middle)"
-                                         % ((bs_ + 15) // 16, nb_, m_, bs_, bs_, dt_),
-                    achieved=ach, peak=PEAK_TFLOPS["fp32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["fp32"],
-                    traffic=measured_traffic("rem_factor_traffic.json"),
-                    algorithmic_flops_per_launch=W["rem_factor"] / 2, avg_ms_per_launch=avg_ms / 2,
-                    factorization_ms=avg_ms)
+        remf = dict(chains="%d slabs x %d blocks of %d x %d (%s)" % (nb_, m_, bs_, bs_, dt_),
+                    form=type(dom.Ared_solver).__name__, ms=avg_ms, algorithmic_gflop=W["rem_factor"] / 1e9,
+                    achieved_tflops=W["rem_factor"] / (avg_ms * 1e-3) / 1e12)
     del dom
     torch.cuda.empty_cache()
     fp64 = None
@@ -334,7 +428,7 @@ def main():
         accuracy_ok = accuracy_ok and err <= 1e-8
         del dom64
         torch.cuda.empty_cache()
-    cfg2 = adj = None
+    cfg2 = adj = mrhs = None
     if not args.no_extras and not args.no_fp64:
         print("bench: constant-coefficient fp64", file=sys.stderr, flush=True)
         dom2, uu2, _ = make_problem(n, p, ppw, "fp64", pde="constant")
@@ -346,6 +440,8 @@ def main():
         torch.cuda.empty_cache()
         print("bench: adjoint", file=sys.stderr, flush=True)
         adj = adjoint_leg()
+        print("bench: multi-rhs", file=sys.stderr, flush=True)
+        mrhs = multirhs_leg()
     if args.calibrate_copy:
         src = torch.ones(64 * 2 ** 20, device="cuda")
         dst = torch.empty_like(src)
@@ -370,7 +466,7 @@ def main():
         "factor_ms": 1e3 * mx["t_factor"], "solve_ms": 1e3 * mx["t_solve"],
         "refinement": refine, "accuracy_ok": accuracy_ok, "stages": stages,
         "fp64": fp64, "speedup_vs_fp64": (fp64["ms_per_step"] / (1e3 * t_step)) if fp64 else None,
-        "cfg2_fp64": cfg2, "adjoint": adj,
+        "cfg2_fp64": cfg2, "adjoint": adj, "multi_rhs": mrhs, "rem_factorization": remf,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -277,6 +277,23 @@ int slab_mdot(int nc, int k, int64_t n, const double* V, int64_t v_cs, int64_t v
 int slab_maxpy(int nc, int k, int64_t n, double alpha, const double* V, int64_t v_cs, int64_t v_ks,
                const double* coef, int64_t coef_cs, double* w, int64_t w_cs, void* stream);
 
+/* Launch probe: HIP events around every launch of one kernel, on the stream
+ * it is launched on, with each launch's algorithmic flops (2 m n k per
+ * product) and bytes (A, B, C read once, C written once), so that a caller
+ * times one kernel inside a larger run without a profiler.  Launches into a
+ * stream under graph capture are not probed.  slab_probe_start creates
+ * 2 * max_launches events (-3 when a probe is running); slab_probe_stop
+ * waits for the recorded events and returns the probed launches, the
+ * launches past max_launches (not timed), the summed event intervals (ms)
+ * and the summed flops and bytes.  One probe per process at a time.  No
+ * counterpart in the reference (measurement of this build's kernels). */
+enum {
+  SLAB_PROBE_GEMM_F32 = 1, /* k_gemm<float, int, true, true>: FP32 MFMA GEMM, 16-byte rows of A, B and C */
+  SLAB_PROBE_GEMM_F64 = 2  /* k_gemm<double, int, false, false>: FP64 MFMA GEMM, strided operands */
+};
+int slab_probe_start(int kernel, int max_launches);
+int slab_probe_stop(int* launches, int* dropped, double* total_ms, double* flops, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

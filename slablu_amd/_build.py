@@ -9,7 +9,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libslabhip.so")
 SOURCES = ["linalg.hip", "leaf.hip", "btri_inv.hip", "btri_lu_factor_f32.hip", "btri_lu_factor_f64.hip", "btri_lu_solve.hip", "getrs_small.hip", "api.hip",
-           "noalloc.hip", "krylov.hip"]
+           "noalloc.hip", "krylov.hip", "probe.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

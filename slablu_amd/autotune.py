@@ -7,7 +7,10 @@ many chains share the GPU, so for a built Domain_Solver and a column count
 the autotuner times every candidate width on the solver's own chains (HIP
 events on the current stream, on a random right-hand side), keeps the
 fastest through slab_set_sweep_tile and records its choices
-(solver.tuning, and a JSON file when a path is given).
+(solver.tuning, and a JSON file when a path is given).  A candidate whose
+solution differs from the built-in rule's by more than a few units of the
+working precision (relative, over all columns) is rejected and recorded as
+such; it is never selected.
 """
 import json
 
@@ -57,22 +60,31 @@ def tune_sweeps(dom, nrhs, candidates=(4, 8, 16, 32), reps=3, path=None):
             work.copy_(f0)
             solver.lusolve_sweep(work)
 
-        timings = {}
+        timings, rejected = {}, {}
+        tol = 1e-10 if dt == torch.float64 else 1e-4
         prev = hip.set_sweep_tile(dt, bs, inv, 0)
         try:
             timings[0] = _time(run, reps)               # the built-in rule
+            ref = work.clone()
+            scale = torch.linalg.norm(ref).item()
             for nr in candidates:
                 if nr > nrhs:
                     continue
                 hip.set_sweep_tile(dt, bs, inv, nr)
-                timings[nr] = _time(run, reps)
+                t = _time(run, reps)
+                diff = torch.linalg.norm(work - ref).item() / max(scale, 1e-300)
+                if not diff <= tol:
+                    rejected[str(nr)] = diff
+                    continue
+                timings[nr] = t
         finally:
             hip.set_sweep_tile(dt, bs, inv, prev)
         best = min(timings, key=timings.get)
         hip.set_sweep_tile(dt, bs, inv, best)
         result[label] = dict(dtype=str(dt).replace("torch.", ""), bs=bs, inverse_factors=inv, nrhs=nrhs,
                              nr=best if best else "built-in", ms=timings[best],
-                             timings={str(k if k else "built-in"): v for k, v in timings.items()})
+                             timings={str(k if k else "built-in"): v for k, v in timings.items()},
+                             rejected=rejected)
     dom.tuning = result
     if path is not None:
         with open(path, "w") as fh:

@@ -5,6 +5,7 @@
 // transposes, sub-blocks and broadcast operands need no copies.
 #include "slab_internal.h"
 #include "wave_ops.h"
+#include "../../include/slab_hip.h"
 
 #include <climits>
 
@@ -324,6 +325,13 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   const bool vc = vec && sizeof(T) == 4 && C.cs == 1 && C.rs % VW == 0 && C.bs % VW == 0 && co % VW == 0 &&
                   al(C.p);
   const bool small = span(m, k, A) < lim && span(k, n, B) < lim && span(m, n, C) < lim;
+  // the launch probe times k_gemm<float, int, true, true> and
+  // k_gemm<double, int, false, false>: 2 m n k flops per product, A, B and
+  // C read once and C written once (C not read when beta is 0)
+  const int pk = (sizeof(T) == 4 && small && vc) ? SLAB_PROBE_GEMM_F32
+                 : (sizeof(T) == 8 && small && !vec) ? SLAB_PROBE_GEMM_F64 : 0;
+  int slot = -1;
+  if (pk != 0 && probe_on(pk)) slot = probe_begin(pk, s);
   if (small && vc)
     hipLaunchKernelGGL((k_gemm<T, int, true, true>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn,
                        ao, bo, co);
@@ -336,7 +344,13 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   else
     hipLaunchKernelGGL((k_gemm<T, int64_t, false, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta,
                        C, tn, ao, bo, co);
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  if (slot >= 0) {
+    const double prods = (double)outer * batch;
+    probe_end(slot, s, 2.0 * prods * m * n * k,
+              prods * sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n));
+  }
+  return e;
 }
 
 template <typename T>

@@ -55,6 +55,16 @@ hipError_t getrs(int batch, int n, int nrhs, bool trans, View<T> LU, const int* 
 }  // namespace slab
 
 namespace slab {
+// Launch probe (probe.hip): probe_on(kernel) is cheap; probe_begin returns
+// a slot (or -1: off, full, or the stream is under capture) and records the
+// start event, probe_end records the end event and the launch's algorithmic
+// flops and bytes.
+bool probe_on(int kernel);
+int probe_begin(int kernel, hipStream_t s);
+void probe_end(int slot, hipStream_t s, double flops, double bytes);
+}  // namespace slab
+
+namespace slab {
 // Batched Krylov products (FP64): out[c*ocs + i] = V_c[i] . w_c for i < k
 // (V_c[i] at V + c*vcs + i*vks, n entries each; ws holds mdot_ws_elems
 // partial sums), and w_c += alpha sum_i coef[c*ccs + i] V_c[i].

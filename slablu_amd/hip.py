@@ -65,6 +65,9 @@ def _load():
         "slab_mdot_ws": ([_int, _int, _i64], ctypes.c_size_t),
         "slab_mdot": ([_int, _int, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp], _int),
         "slab_maxpy": ([_int, _int, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp], _int),
+        "slab_probe_start": ([_int, _int], _int),
+        "slab_probe_stop": ([ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
+                             ctypes.POINTER(_dbl)], _int),
         "slab_leaf_residual": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp],
                                _int),
@@ -446,3 +449,23 @@ def leaf_residual(uc, ub, tables, coefs, p, uu=None):
                                   uu.stride(1) if uu is not None else 0, uu.stride(0) if uu is not None else 0,
                                   _stream()), "slab_leaf_residual")
     return rI, fl
+
+
+PROBE_GEMM_F32, PROBE_GEMM_F64 = 1, 2
+
+
+def probe_start(kernel, max_launches=4096):
+    """Time every launch of one kernel (PROBE_GEMM_F32: k_gemm<float, int,
+    true, true>; PROBE_GEMM_F64: k_gemm<double, int, false, false>) with HIP
+    events on the stream it runs on, until probe_stop."""
+    _check(LIB.slab_probe_start(int(kernel), int(max_launches)), "slab_probe_start")
+
+
+def probe_stop():
+    """dict(launches, dropped, total_ms, flops, bytes) of the probed launches
+    (waits for their events)."""
+    n, d = _int(0), _int(0)
+    ms, fl, by = _dbl(0), _dbl(0), _dbl(0)
+    _check(LIB.slab_probe_stop(ctypes.byref(n), ctypes.byref(d), ctypes.byref(ms), ctypes.byref(fl),
+                               ctypes.byref(by)), "slab_probe_stop")
+    return dict(launches=n.value, dropped=d.value, total_ms=ms.value, flops=fl.value, bytes=by.value)

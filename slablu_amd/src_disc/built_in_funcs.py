@@ -31,6 +31,41 @@ def uu_dir_func_greens_multi(xx, kh, centers):
     return torch.cat(cols, dim=1)
 
 
+def uu_dir_func_mms(xx, Lx, Ly):
+    """sin(Lx x) exp(Ly y), (M, 1): the manufactured solution of the Poisson
+    test (src_disc/built_in_funcs.py:37-41)."""
+    xx = torch.as_tensor(xx)
+    return (torch.sin(Lx * xx[:, 0]) * torch.exp(Ly * xx[:, 1])).unsqueeze(-1)
+
+
+def ff_body_func_mms(xx, Lx, Ly):
+    """-Lap of uu_dir_func_mms: (Lx^2 - Ly^2) sin(Lx x) exp(Ly y)
+    (src_disc/built_in_funcs.py:43-48)."""
+    return (Lx ** 2 - Ly ** 2) * uu_dir_func_mms(xx, Lx, Ly)
+
+
+def uu_dir_pulse(xx, kh):
+    """A Gaussian pulse exp(-200 (y - 1/2)^2) on the left boundary x = 0,
+    zero elsewhere (src_disc/built_in_funcs.py:50-58)."""
+    xx = torch.as_tensor(xx)
+    u = torch.zeros(xx.shape[0], 1, dtype=torch.float64, device=xx.device)
+    left = xx[:, 0] == 0
+    u[left, 0] = torch.exp(-200.0 * (xx[left, 1] - 0.5) ** 2)
+    return u
+
+
+def annulus_sources(count=64, seed=0, inner=1.2, outer=2.0, center=(0.5, 0.5)):
+    """`count` source points drawn uniformly (by area, seeded) from the
+    annulus inner < |x - center| < outer: a list of (x, y) tuples, for the
+    multi-right-hand-side workload (one Green's-function column each)."""
+    g = torch.Generator().manual_seed(seed)
+    u = torch.rand(count, generator=g, dtype=torch.float64)
+    th = torch.rand(count, generator=g, dtype=torch.float64) * (2 * math.pi)
+    r = torch.sqrt(inner ** 2 + u * (outer ** 2 - inner ** 2))
+    return [(float(center[0] + r[i] * math.cos(th[i])), float(center[1] + r[i] * math.sin(th[i])))
+            for i in range(count)]
+
+
 def bfield_constant(xx, kh):
     """c(x) = -kh^2 (src_disc/built_in_funcs.py:79-80)."""
     return -(kh ** 2) * torch.ones(xx.shape[0], device=xx.device, dtype=xx.dtype)

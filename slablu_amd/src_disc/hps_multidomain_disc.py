@@ -216,6 +216,19 @@ class HPS_Multidomain:
         res[:, self.ops.Jc] = r
         return uu, res
 
+    def leaf_values(self, u_face):
+        """Values at every leaf node (nboxes, p^2, nrhs) from the face values
+        (leaf recovery without the residual; no host synchronization)."""
+        L = self.leaf
+        u_bnd = self.faces_to_boxes(u_face).to(L["dtype"]).contiguous()
+        if L["stored"]:
+            uc = self.ops.interior(L["M"], L["ipiv"], u_bnd, None, perm=L.get("perm"))
+        else:
+            uc = torch.empty(self.nboxes, self.ops.ni, u_bnd.shape[-1], dtype=L["dtype"], device=u_bnd.device)
+            for s, e, M, ipiv, perm in self.leaf_chunks():
+                uc[s:e] = self.ops.interior(M, ipiv, u_bnd[s:e], None, perm=perm)
+        return self.ops.assemble_nodes(u_bnd, uc)
+
     def faces_to_boxes(self, x_face):
         """(nface, nrhs) face vector -> (nboxes, 4(p-2), nrhs) per-leaf data."""
         return x_face[self.Iext_d]

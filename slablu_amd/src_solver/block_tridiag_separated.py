@@ -30,14 +30,16 @@ from .. import hip
 
 def separated_period(m, bs, dtype):
     """Period d (piece of d-1 blocks + a separator) for a chain of m blocks,
-    or None when the chain is too short to gain from the split (or its blocks
-    do not run in the one-launch chain kernels)."""
-    if m < 12 or not hip.btri_fused(dtype, bs):
+    or None when the chain is too short to gain from the split."""
+    if m < 12:
         return None
+    # one-launch chains run twisted (both ends towards the middle); wider
+    # blocks run a few batched launches per block step, top-down
+    fused = hip.btri_fused(dtype, bs)
     best = None
     for d in range(3, m // 2 + 1):
         P = -(-m // d)
-        steps = (d - 1) // 2 + 1 + P // 2 + 1
+        steps = ((d - 1) // 2 + 1 + P // 2 + 1) if fused else (d - 1) + P
         waste = P * d - m
         key = (steps + 0.25 * waste, waste)
         if best is None or key < best[0]:
@@ -60,6 +62,11 @@ class SeparatedBlockTridiagonal:
         dev, dt = diag.device, diag.dtype
         l = d - 1
         self.dtype, self.device = dt, dev
+        # pieces and separators factor twisted where their blocks fit the
+        # one-launch chain kernels; inverse factors need the twisted form
+        self.twisted = hip.btri_fused(dt, bs)
+        if not self.twisted:
+            self.inv_factors = False
         # padded chain: identity diagonal blocks, zero couplings
         Dp = torch.zeros(b, mp, bs, bs, dtype=dt, device=dev)
         Dp[:, :m] = diag
@@ -77,7 +84,7 @@ class SeparatedBlockTridiagonal:
         self.pu = Uv[:, :l - 1].contiguous()
         self.pipiv = torch.zeros(nb, l, bs, dtype=torch.int32, device=dev)
         self.pinfo = torch.zeros(nb, dtype=torch.int32, device=dev)
-        hip.btri_factor(self.pd, self.pl, self.pu, self.pipiv, info=self.pinfo, twisted=True)
+        hip.btri_factor(self.pd, self.pl, self.pu, self.pipiv, info=self.pinfo, twisted=self.twisted)
         self.plinv = self.slinv = None
         if self.inv_factors:
             # inverse factors first: the spike sweep below then runs as block
@@ -125,7 +132,7 @@ class SeparatedBlockTridiagonal:
         self.sd, self.sl, self.su = Tn.contiguous(), tsub.contiguous(), tsup.contiguous()
         self.sipiv = torch.zeros(b, P, bs, dtype=torch.int32, device=dev)
         self.sinfo = torch.zeros(b, dtype=torch.int32, device=dev)
-        hip.btri_factor(self.sd, self.sl, self.su, self.sipiv, info=self.sinfo, twisted=True)
+        hip.btri_factor(self.sd, self.sl, self.su, self.sipiv, info=self.sinfo, twisted=self.twisted)
         if self.inv_factors:
             self.slinv = torch.empty_like(self.sd)
             hip.btri_invert_diag(self.sd, self.sipiv, self.slinv)
@@ -140,18 +147,20 @@ class SeparatedBlockTridiagonal:
         """In place on x (periods g0..g1-1, l bs rows, nrhs; any batch stride)."""
         s = slice(g0, g1)
         if self.plinv is not None:
-            hip.btri_solve(self.pd[s], self.pl[s], self.pu[s], None, x, transpose=transpose, twisted=True,
+            hip.btri_solve(self.pd[s], self.pl[s], self.pu[s], None, x, transpose=transpose, twisted=self.twisted,
                            linv=self.plinv[s])
         else:
-            hip.btri_solve(self.pd[s], self.pl[s], self.pu[s], self.pipiv[s], x, transpose=transpose, twisted=True)
+            hip.btri_solve(self.pd[s], self.pl[s], self.pu[s], self.pipiv[s], x, transpose=transpose,
+                           twisted=self.twisted)
 
     def _sep_solve(self, x, transpose, b0, b1):
         s = slice(b0, b1)
         if self.slinv is not None:
-            hip.btri_solve(self.sd[s], self.sl[s], self.su[s], None, x, transpose=transpose, twisted=True,
+            hip.btri_solve(self.sd[s], self.sl[s], self.su[s], None, x, transpose=transpose, twisted=self.twisted,
                            linv=self.slinv[s])
         else:
-            hip.btri_solve(self.sd[s], self.sl[s], self.su[s], self.sipiv[s], x, transpose=transpose, twisted=True)
+            hip.btri_solve(self.sd[s], self.sl[s], self.su[s], self.sipiv[s], x, transpose=transpose,
+                           twisted=self.twisted)
 
     def lusolve_sweep(self, f, b_start=None, b_end=None, transpose=False):
         """Solve in place on f (chains b_start..b_end-1, m*bs rows, nrhs)."""

@@ -74,7 +74,7 @@ def _blockdiag(blocks):
 
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
-                 device=None, refine_tol=1e-8, refine_maxit=12, refine_method="fgmres", refine_restart=10,
+                 device=None, refine_tol=1e-8, refine_maxit=60, refine_method="fgmres", refine_restart=40,
                  check_pivots="warn", on_unconverged="warn", leaf_memory_budget=None, leaf_chunk=None):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
@@ -117,6 +117,8 @@ class Domain_Solver:
         # eager, since capturing costs about two steps
         self.refine_graphs = True
         self._gsteps = {}
+        self._solve_graphs = {}
+        self.tuning = {}
         self._solves_since_build = 0
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.hps_disc(box_geom, h, p, pdo_op)
@@ -194,6 +196,7 @@ class Domain_Solver:
         self.sparse_assembly = sparse_assembly
         self.solver_type = "slabLU"
         self._gsteps = {}  # captured steps read the factors of the previous build
+        self._solve_graphs = {}
         self._solves_since_build = 0
         self.wdt = torch.float32 if self.precision == "mixed" else torch.float64
         t = {}
@@ -693,6 +696,51 @@ class Domain_Solver:
         y.index_add_(0, self.hps.Iext_d.reshape(-1), z.reshape(-1, fb.shape[-1]))
         return (y, w) if return_solve else y
 
+    def _solve_face_device(self, x0):
+        """The FP64 solve from the Dirichlet lift x0 (face vector, Dirichlet
+        values at I_Xtot, zero elsewhere; nrhs columns) with no host
+        synchronization, so that it can be captured: right-hand side -A x0,
+        two-level solve, face solution, leaf recovery.  Returns (face
+        solution, node values (nboxes, p^2, nrhs))."""
+        Ic = self._Ic()
+        ff = -self._face_residual(x0, self.hps.leaf["DtN"])
+        sol = self.solve_reduced(ff.to(self.wdt))
+        x = x0.clone()
+        x[Ic] = sol.to(torch.float64)
+        return x, self.hps.leaf_values(x)
+
+    def solve_graph(self, uu_dir_func):
+        """FP64 solve of all right-hand-side columns at once as one replay of
+        a captured HIP graph (captured at the first call per column count of
+        a factorization; the Dirichlet data are evaluated on the host
+        before).  Returns the node values (nboxes*p^2, nrhs) on the device;
+        the same values as solve() computes."""
+        if self.precision != "fp64":
+            raise ValueError("solve_graph runs the FP64 form (the mixed form refines with host decisions per step)")
+        x0, _ = self.get_rhs(uu_dir_func)
+        nc = x0.shape[1]
+        g = self._solve_graphs.get(nc)
+        if g is None:
+            xin = torch.zeros_like(x0)
+            cur = torch.cuda.current_stream(self.device)
+            side = torch.cuda.Stream(self.device)
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):                   # warm-up outside the capture
+                for _ in range(2):
+                    self._solve_face_device(xin)
+            cur.wait_stream(side)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                xo, uo = self._solve_face_device(xin)
+            g = (graph, xin, xo, uo)
+            self._solve_graphs[nc] = g
+        graph, xin, xo, uo = g
+        xin.copy_(x0)
+        graph.replay()
+        self.last_face = xo
+        self.sol_dev = uo.reshape(-1, nc)
+        return self.sol_dev
+
     def solve(self, uu_dir_func, ff_body_func=None, known_sol=False):
         """Returns (sol_tot at all leaf nodes (cpu), rel_err, true_err,
         resloc_hps, toc_solve) as slablu_solver.py:623-666 does."""
@@ -847,8 +895,12 @@ class Domain_Solver:
         nI = nb * ni
         nc = x_face.shape[-1]
         ntot = nI + Ic.shape[0]
-        # Krylov storage (V and Z: 2 * kmax + 1 vectors per right-hand side)
-        kmax = max(2, min(self.refine_restart, (16 << 30) // max(1, 16 * ntot * nc)))
+        # Krylov storage (V and Z: 2 * kmax + 1 vectors of ntot x nc values):
+        # the restart length is capped by half the free device memory (free
+        # on the device plus what the caching allocator holds unused)
+        free = torch.cuda.mem_get_info(dev)[0] + torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
+        kmax = max(2, min(self.refine_restart, (free // 2) // max(1, 16 * ntot * nc)))
+        self.refine_kmax = kmax
 
         def pack(aI, aC, out):
             out.view(nc, ntot)[:, :nI] = aI.reshape(nI, nc).T
@@ -1041,3 +1093,38 @@ class Domain_Solver:
         xf[Ic] = x
         y = self.hps.apply_A(xf, self.hps.leaf["DtN"].double(), transpose)[Ic]
         return (torch.linalg.norm(y - f) / torch.linalg.norm(f)).item()
+
+    def _apply_ACC(self, x, transpose=False):
+        Ic = self._Ic()
+        xf = torch.zeros(self.hps.nface, x.shape[1], dtype=torch.float64, device=self.device)
+        xf[Ic] = x
+        return self.hps.apply_A(xf, self.hps.leaf["DtN"].double(), transpose)[Ic]
+
+    def norm_ACC(self, iters=30, seed=0):
+        """||A_CC||_2 estimated by power iteration on A_CC^T A_CC (a lower
+        bound that converges from below)."""
+        g = torch.Generator(device=self.device).manual_seed(seed)
+        v = torch.randn(self._Ic().shape[0], 1, dtype=torch.float64, device=self.device, generator=g)
+        v /= torch.linalg.norm(v)
+        s = 0.0
+        for _ in range(iters):
+            w = self._apply_ACC(self._apply_ACC(v), transpose=True)
+            s = torch.linalg.norm(w).item() ** 0.5
+            v = w / torch.linalg.norm(w)
+        return s
+
+    def backward_error(self, nrhs=3, transpose=False, normA=None):
+        """Normwise backward error max_j ||A_CC x_j - f_j|| / (||A_CC|| ||x_j|| +
+        ||f_j||) of the two-level solve for random f (the measure a stable
+        direct solve bounds by a modest multiple of the unit round-off,
+        whatever the conditioning), with the plain relative residuals and
+        the amplification ||x_j|| / ||f_j||."""
+        Ic = self._Ic()
+        nA = self.norm_ACC() if normA is None else normA
+        f = torch.rand(Ic.shape[0], nrhs, dtype=torch.float64, device=self.device)
+        x = self.solve_reduced(f.to(self.wdt), transpose=transpose).double()
+        r = torch.linalg.norm(self._apply_ACC(x, transpose) - f, dim=0)
+        nx, nf = torch.linalg.norm(x, dim=0), torch.linalg.norm(f, dim=0)
+        eta = (r / (nA * nx + nf)).max().item()
+        return dict(backward_error=eta, residual=(r / nf).max().item(), amplification=(nx / nf).max().item(),
+                    norm_A=nA)

@@ -284,14 +284,18 @@ def _btri_dense(diag, sub, sup):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (1, 4, 100), (4, 2, 37), (2, 3, 24), (2, 11, 100)])
+@pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (1, 4, 100), (4, 2, 37), (2, 3, 24), (2, 11, 100),
+                                    (2, 6, 154), (1, 3, 200)])
 @pytest.mark.parametrize("transpose", [False, True])
 @pytest.mark.parametrize("twisted", [False, True])
 def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
     """Block-tridiagonal LU factor + sweeps (sequential, or twisted: both
     chain ends towards block m//2), plain and transposed, against a dense
-    solve."""
+    solve.  Blocks wider than the one-launch chain kernels hold (FP64 above
+    112, FP32 above 160) run the per-step batched launches, top-down only."""
     hip = _hip()
+    if twisted and not hip.btri_fused(dtype, bs):
+        pytest.skip("twisted factors need blocks the one-launch chain kernels hold")
     g = torch.Generator().manual_seed(b * 100 + m * 10 + bs)
     diag0 = torch.randn(b, m, bs, bs, generator=g, dtype=torch.float64) + 3 * torch.eye(bs, dtype=torch.float64)
     sub0 = torch.randn(b, max(m - 1, 1), bs, bs, generator=g, dtype=torch.float64) * 0.3
@@ -630,3 +634,28 @@ def test_getrs_perm_matches_getrs(dtype, n, ld, nrhs):
     torch.cuda.synchronize()
     assert _rel(x2, ref) < 200 * TOL[dtype]
     assert _rel(x2, x1.double().cpu()) < 50 * TOL[dtype]
+
+
+def test_launch_probe_counts_gemm_launches():
+    """The launch probe times exactly the launches of the chosen GEMM kernel:
+    FP32 products with 16-byte rows (k_gemm<float, int, true, true>) are
+    counted with 2 m n k flops each, FP64 and unaligned products are not;
+    launches past the capacity are reported as untimed."""
+    hip = _hip()
+    A = torch.randn(3, 64, 96, device="cuda")
+    B = torch.randn(3, 96, 128, device="cuda")
+    C = torch.empty(3, 64, 128, device="cuda")
+    hip.probe_start(hip.PROBE_GEMM_F32, 4)
+    for _ in range(5):
+        hip.gemm(A, B, C)
+    hip.gemm(A.double(), B.double(), C.double())          # another kernel
+    hip.gemm(A[:, :, 1:], B[:, 1:, :], C)                  # unaligned rows: another variant
+    r = hip.probe_stop()
+    assert r["launches"] == 4 and r["dropped"] == 1
+    assert r["flops"] == 4 * 2 * 3 * 64 * 128 * 96
+    assert r["bytes"] == 4 * 3 * 4 * (64 * 96 + 96 * 128 + 64 * 128)
+    assert r["total_ms"] > 0
+    torch.cuda.synchronize()
+    assert torch.allclose(C, torch.bmm(A[:, :, 1:], B[:, 1:, :]), rtol=1e-4, atol=1e-3)
+    with pytest.raises(RuntimeError):
+        hip.probe_stop()                                     # not running
