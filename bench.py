@@ -28,8 +28,11 @@ Beside it, in the same JSON line:
   roofline      the dominant kernel of the mixed step, the FP32 MFMA GEMM
                 k_gemm<float, int, true, true>: every launch of it in the
                 timed steps bracketed by HIP events on its stream (the
-                library's launch probe), algorithmic flops 2 m n k per product
-                summed over the launches; traffic (HBM bytes per launch) from
+                library's launch probe), algorithmic flops 2 m n k and bytes
+                (A, B, C read once, C written once) per product summed over
+                the launches; bound by HBM when their flop/byte ratio is below
+                the FP32 MFMA ridge point (it is: rank-32 updates dominate);
+                traffic (HBM bytes per launch) from
                 profiles/gemm_f32_traffic.json when present;
   cpu_baseline  the oracle (numpy/scipy FP64) on a bounded sample of the same
                 workload, factor and solve phases, extrapolated.
@@ -395,13 +398,25 @@ def main():
         nl = pr["launches"]
         avg_ms = pr["total_ms"] / nl
         fl, by = pr["flops"] / nl, pr["bytes"] / nl
-        ach = fl / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma",
+        tflops = fl / (avg_ms * 1e-3) / 1e12
+        gbs = by / (avg_ms * 1e-3) / 1e9
+        # arithmetic intensity of the launches against the ridge point of the
+        # FP32 MFMA roof (157.3 TFLOP/s over 8 TB/s = 19.7 flop/byte): most
+        # launches are rank-32 updates (about 8 flop/byte), so the kernel is
+        # bound by HBM
+        ai = fl / by
+        ridge = PEAK_TFLOPS["fp32"] * 1e12 / (PEAK_HBM_GBS * 1e9)
+        hbm = ai < ridge
+        roof = dict(bound="hbm" if hbm else "mfma",
                     kernel="k_gemm<float, int, true, true>: FP32 MFMA GEMM (64 x 64 output tiles, 16-byte rows of "
                            "A, B and C), every launch in the timed steps (leaf LU trailing updates and DtN "
                            "products, slab Schur products)",
-                    achieved=ach, peak=PEAK_TFLOPS["fp32"], unit="TFLOP/s", frac=ach / PEAK_TFLOPS["fp32"],
+                    achieved=gbs if hbm else tflops, peak=PEAK_HBM_GBS if hbm else PEAK_TFLOPS["fp32"],
+                    unit="GB/s" if hbm else "TFLOP/s",
+                    frac=(gbs / PEAK_HBM_GBS) if hbm else (tflops / PEAK_TFLOPS["fp32"]),
                     traffic=measured_traffic("gemm_f32_traffic.json"),
+                    arithmetic_intensity=ai, ridge_point=ridge, achieved_tflops=tflops,
+                    attainable_tflops=min(PEAK_TFLOPS["fp32"], ai * PEAK_HBM_GBS / 1e3),
                     algorithmic_flops_per_launch=fl, algorithmic_bytes_per_launch=by, avg_ms_per_launch=avg_ms,
                     launches_per_step=nl / pr["steps"], untimed_launches=pr["dropped"],
                     timing="HIP events around each launch on its stream (the library's launch probe), timed steps")
