@@ -663,27 +663,33 @@ __global__ __launch_bounds__(256) void k_laswp_panel(int batch, int ncols, int k
 // (w <= W) of one matrix per NT-thread workgroup; thread t holds panel rows
 // t + NT s (s < RP) across all W columns.
 //
-// Rows never move between registers.  Each thread keeps, for each of its
-// rows, the row's current logical position (where the interchanges so far
-// have sent it) and whether it has been a pivot row.  Per column j:
-//  * every thread picks its best not-yet-pivot row (largest |a|, ties to the
-//    smaller logical position, as LAPACK's i?amax), the wavefront reduces it
-//    with DPP, and the lane owning the wavefront's winner publishes the row,
-//    its value and its logical position in LDS;
-//  * after one barrier every thread merges the wavefront candidates into the
-//    pivot (logical position p) and reads the pivot row from LDS; the
-//    interchange j <-> p only relabels logical positions, the pivot row is
-//    marked, and every thread divides its remaining rows by the pivot and
-//    updates them in registers.
-// Candidate buffers alternate between two slots, so writes for column j+2
-// never meet reads of column j.  Rows at or past npiv are updated but never
-// chosen.  The panel moves between HBM and registers through a
-// per-wavefront LDS tile (row-contiguous global accesses); on the way out
-// each row goes to its final logical position.
+// The column loop is bound by instruction issue: at the interface's batch of
+// two matrices one workgroup per CU does all of the work, and in the leaf
+// batches the column loops of all workgroups share the SIMDs.  So few
+// wavefronts hold several rows each, and every step is written to issue as
+// few instructions per wavefront as it can:
+//  * rows never move between registers.  Each thread keeps its rows'
+//    logical positions (where the interchanges so far have sent them) and
+//    whether each is a pivot row already; rows at or past npiv (and padding
+//    rows) are flagged once as never chosen, and they are never interchanged;
+//  * per column every thread folds its candidate rows into one 64-bit key
+//    (largest |a|, ties to the smaller logical position, as LAPACK's i?amax),
+//    the wavefront reduces the keys with DPP, and the one lane holding the
+//    wavefront's winner stores that row to LDS (a branch that one lane
+//    takes, not a blend over all rows);
+//  * after one barrier every wavefront merges the wavefront keys, reads the
+//    pivot row, relabels the two interchanged positions, and scales and
+//    updates its remaining rows in registers.
+// Candidate buffers alternate between two slots, so writes for column j+1
+// never meet reads of column j.  A column without candidates (all NaN) gets
+// a NaN pivot, so everything below it becomes NaN.  The first zero pivot is
+// kept in a register and reported once per panel.  The panel moves between
+// HBM and registers through a per-wavefront LDS tile (row-contiguous global
+// accesses); on the way out each row goes to its final logical position.
 // ---------------------------------------------------------------------------
 template <typename T, int W, int RP, int NT>
 __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, int w, int npiv, View<T> A,
-                                                  int* ipiv, int64_t pbs, int* info) {
+                                                  int* ipiv, int64_t pbs, int* info, bool vld, bool vst) {
   constexpr int NW = NT / 64;
   constexpr int TW = (sizeof(T) == 8 && W > 16) ? 16 : W;  // staging tile width
   static_assert(W % TW == 0 && NT % 64 == 0 && NT <= 1024, "panel shape");
@@ -698,33 +704,54 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
     T x[RP][W];
     int lp[RP];     // logical position of each row
     bool dn[RP];    // pivot row already (or a padding row)
+    bool nc[RP];    // never a candidate: padding, or at or past npiv
 #pragma unroll
     for (int s = 0; s < RP; ++s) {
       lp[s] = tid + NT * s;
       dn[s] = lp[s] >= R;
+      nc[s] = lp[s] >= Rp;
     }
-    // coalesced load: a wavefront moves its 64 rows of slot s in TW-wide
-    // column tiles, 64 / TW rows per load instruction
-    static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
-      constexpr int sl = decltype(sc)::value;
-      const int r0 = NT * sl + 64 * wave;
-      static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
-        constexpr int c0 = decltype(tc)::value * TW;
+    constexpr int VN = 16 / sizeof(T);
+    typedef T vt __attribute__((ext_vector_type(VN)));
+    if (vld) {
+      // 16-byte aligned rows with W columns in bounds: every row of the
+      // panel moves as W / VN vector loads, all of them in flight at once
 #pragma unroll
-        for (int q = 0; q < TW; ++q) {
-          const int e = 64 * q + lane;
-          const int rr = r0 + e / TW, cc = c0 + e % TW;
-          stage[wave][e / TW][e % TW] = *A.at(b, k0 + (rr < rm1 ? rr : rm1), k0 + (cc < wm1 ? cc : wm1));
-        }
-        __syncthreads();
+      for (int s = 0; s < RP; ++s) {
+        const int r = tid + NT * s;
+        const vt* src = reinterpret_cast<const vt*>(A.at(b, k0 + (r < rm1 ? r : rm1), k0));
 #pragma unroll
-        for (int c = 0; c < TW; ++c) {
-          const T v = stage[wave][lane][c];
-          x[sl][c0 + c] = (!dn[sl] & (c0 + c < w)) ? v : T(0);
+        for (int q = 0; q < W / VN; ++q) {
+          const vt v = src[q];
+#pragma unroll
+          for (int e = 0; e < VN; ++e) x[s][q * VN + e] = (!dn[s] & (q * VN + e < w)) ? v[e] : T(0);
         }
-        __syncthreads();
+      }
+    } else {
+      // coalesced load: a wavefront moves its 64 rows of slot s in TW-wide
+      // column tiles, 64 / TW rows per load instruction
+      static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int sl = decltype(sc)::value;
+        const int r0 = NT * sl + 64 * wave;
+        static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
+          constexpr int c0 = decltype(tc)::value * TW;
+  #pragma unroll
+          for (int q = 0; q < TW; ++q) {
+            const int e = 64 * q + lane;
+            const int rr = r0 + e / TW, cc = c0 + e % TW;
+            stage[wave][e / TW][e % TW] = *A.at(b, k0 + (rr < rm1 ? rr : rm1), k0 + (cc < wm1 ? cc : wm1));
+          }
+          __syncthreads();
+  #pragma unroll
+          for (int c = 0; c < TW; ++c) {
+            const T v = stage[wave][lane][c];
+            x[sl][c0 + c] = (!dn[sl] & (c0 + c < w)) ? v : T(0);
+          }
+          __syncthreads();
+        });
       });
-    });
+    }
+    int zc = 0;  // panel column (1-based) of the first exactly zero pivot
     static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
       constexpr int j = decltype(jc)::value;
       if (j < w) {
@@ -733,64 +760,64 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
           const T v = x[s][j];
-          const bool ok = !dn[s] & (lp[s] < Rp) & (v == v);
-          const unsigned long long k = ok ? pivot_key(v, lp[s]) : 0ull;
-          best = umax64(best, k);
+          const bool ok = !(dn[s] | nc[s]) & (v == v);
+          best = umax64(best, ok ? pivot_key(v, lp[s]) : 0ull);
         }
         best = wave_max_u64(best);
-        if (lane == 0) red_k[par][wave] = best;
-        {
-          const int bi = pivot_key_index(best, T(0));
-          bool own = false;
+        if (NW > 1 && lane == 0) red_k[par][wave] = best;
+        const int bi = pivot_key_index(best, T(0));
+        const bool any = best != 0ull;
 #pragma unroll
-          for (int s = 0; s < RP; ++s) own |= !dn[s] & (lp[s] == bi);
-          if (own & (best != 0ull)) {
+        for (int s = 0; s < RP; ++s) {
+          if (any & !dn[s] & (lp[s] == bi)) {
 #pragma unroll
-            for (int c = 0; c < W; ++c) {
-              T v = x[0][c];
-#pragma unroll
-              for (int s = 1; s < RP; ++s) v = bit_blend(v, x[s][c], !dn[s] & (lp[s] == bi));
-              cand[par][wave][c] = v;
-            }
+            for (int c = 0; c < W; ++c) cand[par][wave][c] = x[s][c];
           }
         }
         __syncthreads();
-        // all wavefront keys first (one round of LDS reads), then a tree max
-        unsigned long long mk[NW];
-        int mw[NW];
+        bool valid;
+        int p, wv;
+        if constexpr (NW > 1) {
+          // all wavefront keys first (one round of LDS reads), then a tree max
+          unsigned long long mk[NW];
+          int mw[NW];
 #pragma unroll
-        for (int q = 0; q < NW; ++q) {
-          mk[q] = red_k[par][q];
-          mw[q] = q;
-        }
-#pragma unroll
-        for (int st = 1; st < NW; st *= 2)
-#pragma unroll
-          for (int q = 0; q + st < NW; q += 2 * st) {
-            const bool take = mk[q + st] > mk[q];
-            mk[q] = take ? mk[q + st] : mk[q];
-            mw[q] = take ? mw[q + st] : mw[q];
+          for (int q = 0; q < NW; ++q) {
+            mk[q] = red_k[par][q];
+            mw[q] = q;
           }
-        const bool valid = mk[0] != 0ull;
-        // no candidate (a NaN column): keep the diagonal, results are NaN
-        const int p = valid ? pivot_key_index(mk[0], T(0)) : j;
-        const int wv = mw[0];
+#pragma unroll
+          for (int st = 1; st < NW; st *= 2)
+#pragma unroll
+            for (int q = 0; q + st < NW; q += 2 * st) {
+              const bool take = mk[q + st] > mk[q];
+              mk[q] = take ? mk[q + st] : mk[q];
+              mw[q] = take ? mw[q + st] : mw[q];
+            }
+          valid = mk[0] != 0ull;
+          p = valid ? pivot_key_index(mk[0], T(0)) : j;
+          wv = mw[0];
+        } else {
+          valid = any;
+          p = valid ? bi : j;
+          wv = 0;
+        }
         if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
         T pr[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) pr[c] = cand[par][wv][c];
-#pragma unroll
-        for (int c = 0; c < W; ++c) pr[c] = bit_blend(T(NAN), pr[c], valid);
+        // interchange j <-> p: only the two logical positions change (pivot
+        // rows sit below j, padding and never-chosen rows at or past Rp)
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
           const int o = lp[s];
-          const bool piv = !dn[s] & (o == p);
-          lp[s] = dn[s] ? o : ((o == j) ? p : ((o == p) ? j : o));
+          const bool piv = o == p;
+          lp[s] = piv ? j : ((o == j) ? p : o);
           dn[s] = dn[s] | piv;
         }
-        const T pv = pr[j];
+        const T pv = valid ? pr[j] : T(NAN);
         const bool nz = pv != T(0);
-        if (!nz && tid == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+        zc = (!nz & (zc == 0)) ? j + 1 : zc;
         const T dv = nz ? pv : T(1);
         // FP64: one reciprocal per column (as LAPACK's getf2 scales by
         // 1/pivot); FP32: true division, which refinement depends on
@@ -806,32 +833,58 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
         }
       }
     });
-    // coalesced store through the same tiles, each row to its final position
-    static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
-      constexpr int sl = decltype(sc)::value;
-      const int r0 = NT * sl + 64 * wave;
-      static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
-        constexpr int c0 = decltype(tc)::value * TW;
-        __syncthreads();
+    if ((zc != 0) & (tid == 0) & (info != nullptr))
+      if (info[b] == 0) info[b] = k0 + zc;
+    if (vst) {
+      // each row straight to its final logical position
 #pragma unroll
-        for (int c = 0; c < TW; ++c) stage[wave][lane][c] = x[sl][c0 + c];
-        stage_pos[wave][lane] = lp[sl];
-        __syncthreads();
+      for (int s = 0; s < RP; ++s) {
+        const int r = tid + NT * s;
+        if (r < R) {
+          T* dst = A.at(b, k0 + lp[s], k0);
+          if (w == W) {
 #pragma unroll
-        for (int q = 0; q < TW; ++q) {
-          const int e = 64 * q + lane;
-          const int rr = r0 + e / TW, cc = c0 + e % TW;
-          if (rr < R && cc < w) *A.at(b, k0 + stage_pos[wave][e / TW], k0 + cc) = stage[wave][e / TW][e % TW];
+            for (int q = 0; q < W / VN; ++q) {
+              vt v;
+#pragma unroll
+              for (int e = 0; e < VN; ++e) v[e] = x[s][q * VN + e];
+              reinterpret_cast<vt*>(dst)[q] = v;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+              if (c < w) dst[c] = x[s][c];
+          }
         }
+      }
+    } else {
+      // coalesced store through the same tiles, each row to its final position
+      static_for<0, RP>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int sl = decltype(sc)::value;
+        const int r0 = NT * sl + 64 * wave;
+        static_for<0, W / TW>([&](auto tc) __attribute__((always_inline)) {
+          constexpr int c0 = decltype(tc)::value * TW;
+          __syncthreads();
+  #pragma unroll
+          for (int c = 0; c < TW; ++c) stage[wave][lane][c] = x[sl][c0 + c];
+          stage_pos[wave][lane] = lp[sl];
+          __syncthreads();
+  #pragma unroll
+          for (int q = 0; q < TW; ++q) {
+            const int e = 64 * q + lane;
+            const int rr = r0 + e / TW, cc = c0 + e % TW;
+            if (rr < R && cc < w) *A.at(b, k0 + stage_pos[wave][e / TW], k0 + cc) = stage[wave][e / TW][e % TW];
+          }
+        });
       });
-    });
+    }
     __syncthreads();
   }
 }
 
 // Row interchanges of one panel (rows k0..k0+w-1, 1-based pivots
-// ipiv[k0..k0+w-1] >= row, w <= W) on every column outside the panel, and
-// U12 = L11^{-1} A12 on the columns right of it; one thread per column
+// ipiv[k0..k0+w-1] >= row, w <= W <= 32) on every column outside the panel,
+// and U12 = L11^{-1} A12 on the columns right of it; one thread per column
 // (one wavefront per workgroup).  The rows a panel touches are its own w
 // rows plus the distinct pivot rows below it: each gets a slot, the swap
 // sequence is replayed on slot ids (every lane tracks where its slot ends
@@ -841,6 +894,8 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
 template <typename T, int W>
 __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
                                                   int64_t pbs) {
+  // the w block rows and up to w pivot rows get one slot per lane
+  static_assert(W <= 32, "2 W slots must fit the 64 lanes");
   __shared__ T L[W][W + 1];
   __shared__ int s_piv[W];
   __shared__ int s_slot[W];
@@ -924,19 +979,120 @@ __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, 
 
 static const size_t kPanelLdsBudget = 128 * 1024;
 
-// RP rows per thread in NT-thread register panels
+// Register panels, RP rows per thread in NT-thread workgroups (up to 2048
+// rows).  A panel: its first column, width and kernel shape.
+struct Panel {
+  int k0, w, W, RP, NT;
+};
+
+static inline bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// rows of A (columns from k0) as 16-byte vectors: aligned, row-major
+template <typename T>
+static bool rows_aligned(const View<T>& A, int batch, int k0) {
+  return A.cs == 1 && al16(A.at(0, 0, k0)) && ((A.rs * (int64_t)sizeof(T)) & 15) == 0 &&
+         (batch == 1 || ((A.bs * (int64_t)sizeof(T)) & 15) == 0);
+}
+
 template <typename T, int W, int RP, int NT>
-hipError_t getrf_reg(int batch, int nrows, int ncols, int kbeg, int kend, int npiv, View<T> A, int* ipiv,
-                     int64_t pbs, int* info, hipStream_t s) {
+static void panel_kernel(int batch, int nrows, int ncols, int k0, int w, int npiv, View<T> A, int* ipiv, int64_t pbs,
+                         int* info, hipStream_t s) {
+  // rows of the panel as 16-byte vectors: aligned rows, W columns in
+  // bounds (columns past w are read but masked)
+  const bool vst = rows_aligned(A, batch, k0), vld = vst && k0 + W <= ncols;
+  hipLaunchKernelGGL((k_getf2_reg<T, W, RP, NT>), dim3(grid_batch(batch)), dim3(NT), 0, s, batch, nrows, k0, w, npiv,
+                     A, ipiv, pbs, info, vld, vst);
+}
+
+// the panel kernel of shape (W, RP, NT)
+template <typename T>
+static hipError_t launch_panel(const Panel& P, int batch, int nrows, int ncols, int npiv, View<T> A, int* ipiv,
+                               int64_t pbs, int* info, hipStream_t s) {
+#define SLAB_PANEL(W_, RP_, NT_)                                                                                   \
+  if (P.W == W_ && P.RP == RP_ && P.NT == NT_) {                                                                  \
+    panel_kernel<T, W_, RP_, NT_>(batch, nrows, ncols, P.k0, P.w, npiv, A, ipiv, pbs, info, s);                \
+    return hipGetLastError();                                                                                     \
+  }
+  if constexpr (sizeof(T) == 4) {
+    SLAB_PANEL(32, 4, 64)
+    SLAB_PANEL(32, 4, 128)
+    SLAB_PANEL(32, 4, 256)
+  } else {
+    SLAB_PANEL(32, 2, 128)
+    SLAB_PANEL(32, 2, 256)
+    SLAB_PANEL(16, 4, 256)
+    SLAB_PANEL(32, 1, 256)
+    SLAB_PANEL(32, 1, 512)
+    SLAB_PANEL(16, 2, 512)
+  }
+  SLAB_PANEL(8, 4, 512)
+#undef SLAB_PANEL
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t launch_swap_trsm(int W, int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
+                                   int64_t pbs, hipStream_t s) {
+  const dim3 g(ceil_div(ncols, 64), grid_batch(batch));
+  if (W == 32)
+    hipLaunchKernelGGL((k_swap_trsm<T, 32>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+  else if (W == 16)
+    hipLaunchKernelGGL((k_swap_trsm<T, 16>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+  else
+    hipLaunchKernelGGL((k_swap_trsm<T, 8>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+  return hipGetLastError();
+}
+
+// The panels of an LU of nrows <= 2048 rows.  The panel shape follows the
+// rows still below the diagonal: once they fit the next smaller shape, its
+// panels take over (an FP64 1000-row LU runs 31 16-column panels, then 16
+// 32-column panels).  FP32 and small FP64 batches use few wavefronts with
+// four (two) rows per thread, which shortens each column's chain; large
+// FP64 batches keep lighter wavefronts (more matrices per CU).
+template <typename T>
+static int plan_panels(int batch, int nrows, int nelim, Panel* out) {
+  const bool few = batch <= 64;
+  int n = 0, k0 = 0;
+  while (k0 < nelim) {
+    const int R = nrows - k0;
+    int W, RP, NT, cap;
+    if (R <= 256) {
+      cap = 0;
+      if (sizeof(T) == 4) W = 32, RP = 4, NT = 64;
+      else if (few) W = 32, RP = 2, NT = 128;
+      else W = 32, RP = 1, NT = 256;
+    } else if (R <= 512) {
+      cap = 256;
+      if (sizeof(T) == 4) W = 32, RP = 4, NT = 128;
+      else if (few) W = 32, RP = 2, NT = 256;
+      else W = 32, RP = 1, NT = 512;
+    } else if (R <= 1024) {
+      cap = 512;
+      if (sizeof(T) == 4) W = 32, RP = 4, NT = 256;
+      else if (few) W = 16, RP = 4, NT = 256;
+      else W = 16, RP = 2, NT = 512;
+    } else {
+      cap = 1024, W = 8, RP = 4, NT = 512;
+    }
+    // this shape's panels until the remaining rows fit `cap`
+    const int need = R - cap;
+    int k1 = cap == 0 ? nelim : k0 + ((need + W - 1) / W) * W;
+    k1 = k1 < nelim ? k1 : nelim;
+    for (; k0 < k1; k0 += W) out[n++] = Panel{k0, (k1 - k0) < W ? (k1 - k0) : W, W, RP, NT};
+  }
+  return n;
+}
+
+// right-looking: panel, interchanges + U12 on the other columns, trailing
+// product, one after the other on one stream
+template <typename T>
+static hipError_t getrf_panels(int batch, int nrows, int ncols, int npiv, View<T> A, int* ipiv, int64_t pbs,
+                               int* info, const Panel* P, int np, hipStream_t s) {
   hipError_t e;
-  for (int k0 = kbeg; k0 < kend; k0 += W) {
-    const int w = (kend - k0) < W ? (kend - k0) : W;
-    hipLaunchKernelGGL((k_getf2_reg<T, W, RP, NT>), dim3(grid_batch(batch)), dim3(NT), 0, s, batch, nrows, k0, w,
-                       npiv, A, ipiv, pbs, info);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_swap_trsm<T, W>), dim3(ceil_div(ncols, 64), grid_batch(batch)), dim3(64), 0, s, batch,
-                       ncols, k0, w, A, ipiv, pbs);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+  for (int i = 0; i < np; ++i) {
+    const int k0 = P[i].k0, w = P[i].w;
+    if ((e = launch_panel<T>(P[i], batch, nrows, ncols, npiv, A, ipiv, pbs, info, s)) != hipSuccess) return e;
+    if ((e = launch_swap_trsm<T>(P[i].W, batch, ncols, k0, w, A, ipiv, pbs, s)) != hipSuccess) return e;
     const int mr = nrows - k0 - w, nr = ncols - k0 - w;
     if (mr > 0 && nr > 0) {
       e = gemm<T>(batch, mr, nr, w, T(-1), A.sub(k0 + w, k0), A.sub(k0, k0 + w), T(1), A.sub(k0 + w, k0 + w), s);
@@ -951,42 +1107,10 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
                  hipStream_t s) {
   if (batch <= 0 || nelim <= 0) return hipSuccess;
   if (nelim > npiv || npiv > nrows || nelim > ncols) return hipErrorInvalidValue;
-  // panels held in registers, RP rows per thread: batched small matrices in
-  // 256-thread workgroups (several per CU), up to 1024 rows in 512-thread
-  // workgroups (two wavefronts per SIMD), four rows per thread up to 2048
-  // The panel shape follows the rows still below the diagonal: once they
-  // fit the next smaller shape, its wider panels take over (a 1000-row LU
-  // runs 31 16-column panels, then 16 32-column panels).
   if (nrows <= 2048) {
-    hipError_t e;
-    int k0 = 0;
-    while (k0 < nelim) {
-      const int R = nrows - k0;
-      // first column from which the remaining rows fit `cap`, on this
-      // phase's panel grid (multiples of W from k0), capped at nelim
-      auto until = [&](int cap, int W) {
-        const int need = R - cap;  // columns to eliminate before R <= cap
-        const int c = k0 + ((need + W - 1) / W) * W;
-        return c < nelim ? c : nelim;
-      };
-      int k1;
-      if (R <= 256) {
-        k1 = nelim;
-        e = getrf_reg<T, 32, 1, 256>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
-      } else if (R <= 512) {
-        k1 = until(256, 32);
-        e = getrf_reg<T, 32, 1, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
-      } else if (R <= 1024) {
-        k1 = until(512, 16);
-        e = getrf_reg<T, 16, 2, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
-      } else {
-        k1 = until(1024, 8);
-        e = getrf_reg<T, 8, 4, 512>(batch, nrows, ncols, k0, k1, npiv, A, ipiv, pbs, info, s);
-      }
-      if (e != hipSuccess) return e;
-      k0 = k1;
-    }
-    return hipSuccess;
+    Panel P[2048];
+    const int np = plan_panels<T>(batch, nrows, nelim, P);
+    return getrf_panels<T>(batch, nrows, ncols, npiv, A, ipiv, pbs, info, P, np, s);
   }
   // taller panels: LDS-resident panel, generic interchanges
   int nb = 32;

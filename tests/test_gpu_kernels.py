@@ -217,6 +217,59 @@ def test_getrf_rectangular_and_bordered(dtype):
             assert _rel(Mc[:, ni:, ni:], S) < 100 * TOL[dtype]
 
 
+def _getrf_residual(A0, M, piv, nelim):
+    """||P A - L U|| / ||A|| for getrf's packed output of an nrows x ncols
+    matrix with nelim eliminated columns: L = the unit-lower multipliers in
+    the first nelim columns (identity below for the rest), U = the rest of
+    M, whose trailing rows are the Schur complement of the bordered rows."""
+    A0 = np.asarray(A0, dtype=np.float64)
+    M = np.asarray(M, dtype=np.float64)
+    nrows, ncols = M.shape
+    PA = A0.copy()
+    for i, p in enumerate(piv[:nelim]):
+        if p != i:
+            PA[[i, p]] = PA[[p, i]]
+    L = np.eye(nrows)
+    L[:, :nelim] += np.tril(M[:, :nelim], -1)
+    U = M.copy()
+    U[:, :nelim] = np.triu(M[:, :nelim])
+    return np.linalg.norm(L @ U - PA) / np.linalg.norm(A0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("b,nrows,ncols,nelim,npiv", [(2, 1000, 1000, 1000, 1000), (1, 300, 300, 300, 300),
+                                                       (3, 700, 780, 700, 700), (2, 600, 650, 450, 500),
+                                                       (1, 1500, 1500, 1500, 1500), (80, 400, 480, 400, 400),
+                                                       (80, 300, 320, 260, 280)])
+def test_getrf_panel_phases_reconstruct(dtype, b, nrows, ncols, nelim, npiv):
+    """getrf's register panels change shape with the rows left below the
+    diagonal (8/16/32 columns; few heavy wavefronts for FP32 and small FP64
+    batches, more lighter ones for large FP64 batches) and move rows as
+    16-byte vectors where aligned.  On square, rectangular (extra columns)
+    and bordered (rows past npiv never pivot) shapes at both batch sizes:
+    P A = L U with the bordered Schur complement, to the precision's
+    rounding; no pivot row at or past npiv; FP64 square pivots as LAPACK's."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(nrows + 7 * ncols + b)
+    M0 = torch.randn(b, nrows, ncols, generator=g, dtype=torch.float64)
+    M = M0.to("cuda", dtype).contiguous()
+    ipiv = torch.zeros(b, nelim, dtype=torch.int32, device="cuda")
+    info = torch.zeros(b, dtype=torch.int32, device="cuda")
+    hip.getrf(M, ipiv, nelim=nelim, npiv=npiv, info=info)
+    torch.cuda.synchronize()
+    Mc, pc = M.double().cpu(), ipiv.cpu()
+    assert (info.cpu() == 0).all()
+    tol = 1e-13 if dtype == torch.float64 else 2e-5
+    for k in range(0, b, max(1, b // 4)):
+        assert (pc[k].numpy() - 1 < npiv).all()
+        res = _getrf_residual(M0[k].to(dtype).double().numpy(), Mc[k].numpy(), pc[k].numpy() - 1, nelim)
+        assert res < tol * max(1, nrows / 100)
+        if dtype == torch.float64 and nrows == ncols == nelim == npiv:
+            lu_ref, piv_ref = scipy.linalg.lu_factor(M0[k].numpy())
+            assert np.array_equal(pc[k].numpy() - 1, piv_ref)
+            assert np.abs(Mc[k].numpy() - lu_ref).max() < 1e-9 * max(1, np.abs(lu_ref).max())
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("trans", [False, True])
 def test_getrs(dtype, trans):
