@@ -26,15 +26,31 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 // (along n) instead of single elements.  VC (with VEC): C row-major and
 // aligned too; the C tile is read as row vectors before the K loop and the
 // result leaves through an LDS tile, so C moves as 16-byte row vectors.
+// tiles > 0 (XCD-aware order, one-dimensional grid of tiles * 8 *
+// ceil(batch / 8) workgroups): workgroups go to the eight XCDs round robin,
+// so workgroup L runs on XCD L % 8; here the XCD's i-th workgroup (i = L / 8)
+// takes tile i % tiles of matrix (L % 8) + 8 (i / tiles).  All tiles of one
+// matrix then run on one XCD, one matrix after the other, and the operand
+// rows and columns that several output tiles share are read from HBM once
+// into that XCD's L2.  tiles = 0: tile blockIdx.x of matrices blockIdx.y,
+// blockIdx.y + gridDim.y, ...
 template <typename T, typename I, bool VEC, bool VC>
 __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B,
-                                              T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co) {
+                                              T beta, View<T> C, int tiles_n, int64_t ao, int64_t bo, int64_t co,
+                                              int tiles) {
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int VW = VEC ? 16 / (int)sizeof(T) : 1;
   typedef T vec_t __attribute__((ext_vector_type(VW)));
   __shared__ T As[GBK][GBM + 1];
   __shared__ T Bs[GBK][GBN + 1];
-  const int tile = blockIdx.x;
+  int tile = blockIdx.x, bfirst = blockIdx.y, bstep = gridDim.y;
+  if (tiles > 0) {
+    const int xi = blockIdx.x >> 3;
+    tile = xi % tiles;
+    bfirst = (blockIdx.x & 7) + 8 * (xi / tiles);
+    bstep = batch;  // one matrix per workgroup
+    if (bfirst >= batch) return;
+  }
   const int m0 = (tile / tiles_n) * GBM, n0 = (tile % tiles_n) * GBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -91,7 +107,7 @@ __global__ __launch_bounds__(256) void k_gemm(int batch, int m, int n, int k, T 
     const int col = n0 + wn * 32 + j * 16 + (lane & 15);
     c_col[j] = (I)(col < nm1 ? col : nm1) * (I)C.cs;
   }
-  for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+  for (int b = bfirst; b < batch; b += bstep) {
     // second batch level (grid z): whole-operand offsets
     const T* Ab = A.p + blockIdx.z * ao + b * A.bs;
     const T* Bb = B.p + blockIdx.z * bo + b * B.bs;
@@ -308,6 +324,13 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   if (k < 0) k = 0;
   const int tn = ceil_div(n, GBN), tm = ceil_div(m, GBM);
   dim3 grid(tm * tn, grid_batch(batch), outer);
+  // large batches of several-tile products: XCD-aware order (a matrix's
+  // tiles on one XCD); small batches keep every XCD busy in the plain order
+  int xt = 0;
+  if (outer == 1 && batch >= 16 && tm * tn > 1 && (int64_t)tm * tn * 8 * ceil_div(batch, 8) < INT_MAX) {
+    xt = tm * tn;
+    grid = dim3((unsigned)(xt * 8 * ceil_div(batch, 8)), 1, 1);
+  }
   // 32-bit offsets inside each matrix when every element offset fits
   auto span = [](int rows, int cols, const View<T>& V) {
     const int64_t r = rows > 0 ? rows - 1 : 0, c = cols > 0 ? cols - 1 : 0;
@@ -334,16 +357,16 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   if (pk != 0 && probe_on(pk)) slot = probe_begin(pk, s);
   if (small && vc)
     hipLaunchKernelGGL((k_gemm<T, int, true, true>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C, tn,
-                       ao, bo, co);
+                       ao, bo, co, xt);
   else if (small && vec)
     hipLaunchKernelGGL((k_gemm<T, int, true, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C,
-                       tn, ao, bo, co);
+                       tn, ao, bo, co, xt);
   else if (small)
     hipLaunchKernelGGL((k_gemm<T, int, false, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta, C,
-                       tn, ao, bo, co);
+                       tn, ao, bo, co, xt);
   else
     hipLaunchKernelGGL((k_gemm<T, int64_t, false, false>), grid, dim3(256), 0, s, batch, m, n, k, alpha, A, B, beta,
-                       C, tn, ao, bo, co);
+                       C, tn, ao, bo, co, xt);
   const hipError_t e = hipGetLastError();
   if (slot >= 0) {
     const double prods = (double)outer * batch;
