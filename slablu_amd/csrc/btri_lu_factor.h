@@ -128,8 +128,378 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int bs, ChainSet<T> c0, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA form of the factor kernel for blocks that fit LDS three times over
+// (FP32, bs <= 112).  Ufac_i and Lfac_i stay in LDS after their solves, so
+// block i+1's Schur product Lfac_i Ufac_i reads no global memory; both
+// triangular solves run as blocked substitutions whose off-diagonal updates
+// are MFMA tiles; every global transfer moves 16-byte vectors.
+// ---------------------------------------------------------------------------
+
+// Row stride of the LDS Ufac / Lfac^T blocks: at least bs and 16 mod 32, so
+// the two rows a half-wavefront reads in one MFMA operand load fall in
+// distinct banks.
+__host__ __device__ inline int lu_mfma_ldx(int bs) { return bs <= 16 ? 16 : ((bs - 16 + 31) / 32) * 32 + 16; }
+
+template <typename T>
+size_t lu_mfma_lds(int bs) {
+  const size_t bp = 16 * (size_t)tile_rt(bs);
+  return sizeof(T) * (bp * (bp + 1) + 2 * (size_t)bs * lu_mfma_ldx(bs)) + 2 * sizeof(int) * bs;
+}
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float4 t;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double2 t;
+  static constexpr int n = 2;
+};
+
+__device__ inline bool al16d(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// S (row stride ld) <- the bs x bs row-major block src
+template <typename T>
+__device__ inline void load_block_v(T* S, int ld, const T* src, int bs) {
+  typedef typename Vec16<T>::t vt;
+  constexpr int V = Vec16<T>::n;
+  if ((bs % V) == 0 && al16d(src)) {
+    const int nv = bs / V, n = bs * nv;
+    const float rnv = 1.0f / (float)nv;
+    const vt* s4 = reinterpret_cast<const vt*>(src);
+    for (int e0 = threadIdx.x; e0 < n; e0 += 256 * 8) {
+      vt v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 256 * u;
+        v[u] = s4[e < n ? e : n - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 256 * u;
+        if (e < n) {
+          const int r = idiv(e, rnv), c = V * (e - r * nv);
+          const T* pv = reinterpret_cast<const T*>(&v[u]);
+#pragma unroll
+          for (int t = 0; t < V; ++t) S[r * ld + c + t] = pv[t];
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    stage_block<T>(S, src, bs, 1.0f / (float)bs, ld);
+  }
+}
+
+// dst (bs x bs row-major) <- S (row stride ld)
+template <typename T>
+__device__ inline void store_block_v(T* dst, const T* S, int ld, int bs) {
+  typedef typename Vec16<T>::t vt;
+  constexpr int V = Vec16<T>::n;
+  if ((bs % V) == 0 && al16d(dst)) {
+    const int nv = bs / V, n = bs * nv;
+    const float rnv = 1.0f / (float)nv;
+    vt* d4 = reinterpret_cast<vt*>(dst);
+    for (int e = threadIdx.x; e < n; e += 256) {
+      const int r = idiv(e, rnv), c = V * (e - r * nv);
+      vt v;
+      T* pv = reinterpret_cast<T*>(&v);
+#pragma unroll
+      for (int t = 0; t < V; ++t) pv[t] = S[r * ld + c + t];
+      d4[e] = v;
+    }
+  } else {
+    const float rbs = 1.0f / (float)bs;
+    for (int e = threadIdx.x; e < bs * bs; e += 256) {
+      const int r = idiv(e, rbs);
+      dst[e] = S[r * ld + e - r * bs];
+    }
+  }
+}
+
+// XU[r][c] = U[Q[r]][c] (rows of the coupling block through the pivot
+// permutation) and XL[r][c] = L[c][r] (the coupling block transposed), row
+// stride ldx
+template <typename T>
+__device__ inline void gather_couplings(T* XU, T* XL, int ldx, const T* U, const T* L, const int* Q, int bs) {
+  typedef typename Vec16<T>::t vt;
+  constexpr int V = Vec16<T>::n;
+  if ((bs % V) == 0 && al16d(U) && al16d(L)) {
+    const int nv = bs / V, n = bs * nv;
+    const float rnv = 1.0f / (float)nv;
+    for (int e0 = threadIdx.x; e0 < n; e0 += 256 * 4) {
+      vt u[4], l[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + 256 * q;
+        const int ec = e < n ? e : n - 1;
+        const int r = idiv(ec, rnv), cv = ec - r * nv;
+        u[q] = *reinterpret_cast<const vt*>(U + (int64_t)Q[r] * bs + V * cv);
+        l[q] = *reinterpret_cast<const vt*>(L + (int64_t)r * bs + V * cv);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + 256 * q;
+        if (e < n) {
+          const int r = idiv(e, rnv), c = V * (e - r * nv);
+          const T* pu = reinterpret_cast<const T*>(&u[q]);
+          const T* pl = reinterpret_cast<const T*>(&l[q]);
+#pragma unroll
+          for (int t = 0; t < V; ++t) {
+            XU[r * ldx + c + t] = pu[t];
+            XL[(c + t) * ldx + r] = pl[t];
+          }
+        }
+      }
+    }
+  } else {
+    const float rbs = 1.0f / (float)bs;
+    for (int e = threadIdx.x; e < bs * bs; e += 256) {
+      const int r = idiv(e, rbs), c = e - r * bs;
+      XU[r * ldx + c] = U[(int64_t)Q[r] * bs + c];
+      XL[c * ldx + r] = L[e];
+    }
+  }
+  __syncthreads();
+}
+
+// U[r][c] = XU[r][c], L[r][c] = XL[c][r]
+template <typename T>
+__device__ inline void store_couplings(T* U, T* L, const T* XU, const T* XL, int ldx, int bs) {
+  typedef typename Vec16<T>::t vt;
+  constexpr int V = Vec16<T>::n;
+  if ((bs % V) == 0 && al16d(U) && al16d(L)) {
+    const int nv = bs / V, n = bs * nv;
+    const float rnv = 1.0f / (float)nv;
+    for (int e = threadIdx.x; e < n; e += 256) {
+      const int r = idiv(e, rnv), c = V * (e - r * nv);
+      vt u, l;
+      T* pu = reinterpret_cast<T*>(&u);
+      T* pl = reinterpret_cast<T*>(&l);
+#pragma unroll
+      for (int t = 0; t < V; ++t) {
+        pu[t] = XU[r * ldx + c + t];
+        pl[t] = XL[(c + t) * ldx + r];
+      }
+      *reinterpret_cast<vt*>(U + (int64_t)r * bs + c) = u;
+      *reinterpret_cast<vt*>(L + (int64_t)r * bs + c) = l;
+    }
+  } else {
+    const float rbs = 1.0f / (float)bs;
+    for (int e = threadIdx.x; e < bs * bs; e += 256) {
+      const int r = idiv(e, rbs), c = e - r * bs;
+      U[e] = XU[r * ldx + c];
+      L[e] = XL[c * ldx + r];
+    }
+  }
+}
+
+// S[0:bs, 0:bs] -= XL^T XU (both bs x bs, row stride ldx, RT = ceil(bs/16)
+// tiles per side): wavefront w owns column tiles w and w + 4 and every row
+// tile, so each K step reads one XU fragment and RT XL fragments, then
+// issues RT independent MFMAs.
+template <typename T, int RT>
+__device__ __attribute__((always_inline)) void schur_mfma(T* S, int ld, const T* XL, const T* XU, int ldx, int bs) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bm1 = bs - 1;
+  const int i16 = lane & 15, kq = lane >> 4;
+  int aoff[RT];
+#pragma unroll
+  for (int j = 0; j < RT; ++j) {
+    const int r = j * 16 + i16;
+    aoff[j] = r < bm1 ? r : bm1;
+  }
+  for (int ct = wave; ct < RT; ct += 4) {
+    acc_t acc[RT];
+#pragma unroll
+    for (int j = 0; j < RT; ++j) acc[j] = acc_t{0, 0, 0, 0};
+    const int col = ct * 16 + i16;
+    const int colc = col < bm1 ? col : bm1;
+#pragma unroll 2
+    for (int k0 = 0; k0 < bs; k0 += 4) {
+      const int k = k0 + kq;
+      const bool kin = k < bs;
+      const int kr = (kin ? k : bm1) * ldx;
+      T av[RT];
+      const T bv = XU[kr + colc];
+#pragma unroll
+      for (int j = 0; j < RT; ++j) av[j] = XL[kr + aoff[j]];
+      const T b = kin ? bv : T(0);
+#pragma unroll
+      for (int j = 0; j < RT; ++j) acc[j] = Mfma<T>::mma(kin ? av[j] : T(0), b, acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < RT; ++j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = j * 16 + Mfma<T>::row(lane, q);
+        if ((r < bs) & (col < bs)) S[r * ld + col] -= acc[j][q];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// One 16-row diagonal block (rows d0 .. d0+tb-1) of M X = X for column c,
+// by substitution in registers (M lower, read through `at`; rows past n-1
+// are read clamped and never stored).
+template <typename T, bool UNIT, typename At>
+__device__ inline void diag_block_solve(At at, T* X, int ldx, int d0, int tb, int nm1, int c) {
+  T x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) x[j] = X[(d0 + (j < tb ? j : tb - 1)) * ldx + c];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int jj = (d0 + j) < nm1 ? (d0 + j) : nm1;
+    const T xj = UNIT ? x[j] : x[j] / at(jj, jj);
+    x[j] = xj;
+#pragma unroll
+    for (int r = j + 1; r < 16; ++r) x[r] -= at((d0 + r) < nm1 ? (d0 + r) : nm1, jj) * xj;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (j < tb) X[(d0 + j) * ldx + c] = x[j];
+}
+
+// Block row d0 .. d0+15 of X -= M[d0.., 0..d0) X[0..d0, :] for column
+// tiles ct0, ct0 + 2, ct0 + 4, ct0 + 6 (those below RT): one M fragment
+// and four X fragments per K step feed four independent MFMAs.
+template <typename T, int RT, typename At>
+__device__ __attribute__((always_inline)) void block_row_update(At at, T* X, int ldx, int n, int d0, int ct0) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int nm1 = n - 1;
+  const int rowc = (d0 + i16) < nm1 ? (d0 + i16) : nm1;
+  int coff[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int col = (ct0 + 2 * q) * 16 + i16;
+    coff[q] = col < nm1 ? col : nm1;
+  }
+  acc_t acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
+  // d0 is a multiple of 16: every K step is in range
+#pragma unroll 4
+  for (int k0 = 0; k0 < d0; k0 += 4) {
+    const int k = k0 + kq;
+    const T a = at(rowc, k);
+    T bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bv[q] = X[k * ldx + coff[q]];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = Mfma<T>::mma(a, bv[q], acc[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int col = (ct0 + 2 * q) * 16 + i16;
+    if ((ct0 + 2 * q < RT) & (col < n)) {
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int r = d0 + Mfma<T>::row(lane, qq);
+        if (r < n) X[r * ldx + col] -= acc[q][qq];
+      }
+    }
+  }
+}
+
+// X1 <- M1^{-1} X1 (M1 unit lower, through at1) and X2 <- M2^{-1} X2 (M2
+// lower, through at2), n x n each with row stride ldx, RT = ceil(n/16) <= 8
+// block rows, left-looking: per block row, the MFMA product with the rows
+// already solved (wavefronts 0-1 X1, 2-3 X2, alternate column tiles), then
+// the diagonal block by substitution (wavefronts 0-1 X1, 2-3 X2, a thread
+// per column).
+template <typename T, int RT, typename A1, typename A2>
+__device__ __attribute__((always_inline)) void tri_solve2_mfma(A1 at1, A2 at2, int n, T* X1, T* X2, int ldx) {
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int nm1 = n - 1;
+  for (int kb = 0; kb < RT; ++kb) {
+    const int d0 = kb << 4;
+    const int tb = (n - d0) < 16 ? (n - d0) : 16;
+    if (kb > 0) {
+      if (wave < 2) block_row_update<T, RT>(at1, X1, ldx, n, d0, wave & 1);
+      else block_row_update<T, RT>(at2, X2, ldx, n, d0, wave & 1);
+      __syncthreads();
+    }
+    const int c = tid & 127;
+    if (c < n) {
+      if (tid < 128) diag_block_solve<T, true>(at1, X1, ldx, d0, tb, nm1, c);
+      else diag_block_solve<T, false>(at2, X2, ldx, d0, tb, nm1, c);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int RT>
+__global__ __launch_bounds__(256) void k_btri_lu_factor_mfma(int bs, ChainSet<T> c0, ChainSet<T> c1) {
+  constexpr int BP = 16 * RT;
+  static_assert(RT <= 7, "Schur product tiles");
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw);  // BP x (BP+1), padded diagonal block
+  const int ldx = lu_mfma_ldx(bs);
+  T* XU = S + BP * (BP + 1);              // Ufac of the current block
+  T* XL = XU + bs * ldx;                  // Lfac^T of the current block
+  int* piv = reinterpret_cast<int*>(XL + bs * ldx);
+  int* Q = piv + bs;
+  __shared__ __align__(16) int wsp[6 * 16 + 2 * 16 * (int)(sizeof(T) / sizeof(int))];
+  const int ld = BP + 1;
+  auto Lat = [&](int r, int k) -> T { return S[r * ld + k]; };
+  auto UTat = [&](int r, int k) -> T { return S[k * ld + r]; };
+  const int nt = c0.nchains + c1.nchains;
+  for (int g = blockIdx.x; g < nt; g += gridDim.x) {
+    const bool sec = g >= c0.nchains;
+    const ChainSet<T> a = sec ? c1 : c0;
+    const int64_t b = sec ? g - c0.nchains : g;
+    const int m = a.m;
+    for (int i = 0; i < m; ++i) {
+      T* Di = a.diag + b * a.dbs + (int64_t)i * a.dis;
+      load_block_v<T>(S, ld, Di, bs);
+      // Lfac_{i-1} Ufac_{i-1}, both still in LDS from the previous block
+      if (i > 0) schur_mfma<T, RT>(S, ld, XL, XU, ldx, bs);
+      lu_lds_blocked<T, RT>(S, bs, ld, piv, wsp, a.info, b);
+      store_block_v<T>(Di, S, ld, bs);
+      int* Pi = a.ipiv + b * a.pbs + (int64_t)i * a.pis;
+      for (int r = threadIdx.x; r < bs; r += 256) Pi[r] = piv[r] + 1;
+      if (i < m - 1 || a.open) {
+        T* Ui = a.sup + b * a.ubs + (int64_t)i * a.uis;
+        T* Li = a.sub + b * a.lbs + (int64_t)i * a.lis;
+        piv_to_perm(piv, bs, Q);
+        gather_couplings<T>(XU, XL, ldx, Ui, Li, Q, bs);
+        // Ufac = L^{-1} (P sup_i);  U^T Lfac^T = sub_i^T
+        tri_solve2_mfma<T, RT>(Lat, UTat, bs, XU, XL, ldx);
+        store_couplings<T>(Ui, Li, XU, XL, ldx, bs);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+bool lu_mfma_ok(int bs) {
+  return sizeof(T) == 4 && tile_rt(bs) <= 7 && lu_mfma_lds<T>(bs) + 1024 <= kLdsMax;
+}
+
 template <typename T, int RT>
 hipError_t launch_lu_factor(int bs, const ChainSet<T>& c0, const ChainSet<T>& c1, hipStream_t s) {
+  if constexpr (RT <= 7 && sizeof(T) == 4) {
+    if (lu_mfma_ok<T>(bs)) {
+      const size_t lds = lu_mfma_lds<T>(bs);
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_factor_mfma<T, RT>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      const int nt = c0.nchains + c1.nchains;
+      if (nt <= 0) return hipSuccess;
+      hipLaunchKernelGGL((k_btri_lu_factor_mfma<T, RT>), dim3(nt < 65535 ? nt : 65535), dim3(256), lds, s, bs, c0,
+                         c1);
+      return hipGetLastError();
+    }
+  }
   const int xc = lu_factor_xc<T>(bs);
   const size_t lds = lu_factor_lds<T>(bs, xc);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_btri_lu_factor<T, RT>),

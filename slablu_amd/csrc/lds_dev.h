@@ -426,6 +426,52 @@ __device__ inline void panel_slots(const int* piv, int k0, int nb, int* s_slot, 
   __builtin_amdgcn_wave_barrier();
 }
 
+// S[kk.., kk..] -= S[kk.., k0..k0+nb) S[k0..k0+nb), kk..] with kk = k0 + nb a
+// multiple of 16 (n <= 16RT, row stride ld): wavefront w takes column tiles
+// w, w + 4, ... from kk's tile on and every row tile (rows above kk multiply
+// zeros), so each K step reads one fragment of the panel's row block and RT
+// of its column block, then issues RT independent MFMAs.
+template <typename T, int RT>
+__device__ __attribute__((always_inline)) void lds_panel_update(T* S, int n, int ld, int k0, int nb) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int kk = k0 + nb, t0 = kk >> 4, nm1 = n - 1;
+  for (int ct = t0 + wave; ct < RT; ct += 4) {
+    const int col = ct * 16 + i16;
+    const int colc = col < nm1 ? col : nm1;
+    acc_t acc[RT];
+#pragma unroll
+    for (int j = 0; j < RT; ++j) acc[j] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int k = 4 * ks + kq;
+      const bool kin = k < nb;
+      const int kc = k0 + (kin ? k : nb - 1);
+      const T bv = S[kc * ld + colc];
+      T av[RT];
+#pragma unroll
+      for (int j = 0; j < RT; ++j) {
+        const int r = j * 16 + i16;
+        av[j] = S[(r < nm1 ? r : nm1) * ld + kc];
+      }
+      const T b = kin ? bv : T(0);
+#pragma unroll
+      for (int j = 0; j < RT; ++j) acc[j] = Mfma<T>::mma((kin & (j >= t0)) ? av[j] : T(0), b, acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < RT; ++j) {
+      if (j >= t0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = j * 16 + Mfma<T>::row(lane, q);
+          if ((r < n) & (col < n)) S[r * ld + col] -= acc[j][q];
+        }
+      }
+    }
+  }
+}
+
 // Blocked in-place LU with partial pivoting of the n x n matrix S in LDS
 // (row stride ld >= 16RT + 1, padded to 16RT rows and columns, n <= 16RT).
 // Each 16-column panel is factored by wavefront 0 alone with the panel in
@@ -439,11 +485,10 @@ __device__ inline void panel_slots(const int* piv, int k0, int nb, int* s_slot, 
 // of padding rows/columns enter as zeros).  Four barriers per panel.
 // piv[k] is 0-based.  wsp: LDS workspace of 6 * 16 ints + 32 T.
 template <typename T, int RT>
-__device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* info, int64_t b) {
+__device__ __attribute__((always_inline)) void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* info, int64_t b) {
   constexpr int NB = 16;
   constexpr int RPW = (16 * RT + 63) / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tx = tid & 15, ty = tid >> 4;
   int* s_slot = wsp;                 // NB
   int* s_row = wsp + NB;             // 2 NB
   int* s_src = wsp + 3 * NB;         // 2 NB
@@ -589,39 +634,7 @@ __device__ void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* inf
     }
     __syncthreads();
     // trailing update: rows and columns k0+nb .. n-1
-    {
-      const int kk = k0 + nb;
-      if (kk < n) {
-        T acc[RT][RT];
-#pragma unroll
-        for (int a = 0; a < RT; ++a)
-#pragma unroll
-          for (int q = 0; q < RT; ++q) acc[a][q] = T(0);
-        for (int j = 0; j < nb; ++j) {
-          T av[RT], bv[RT];
-#pragma unroll
-          for (int a = 0; a < RT; ++a) {
-            const int r = ty + 16 * a;
-            const T v = S[r * ld + k0 + j];
-            av[a] = ((r >= kk) & (r < n)) ? v : T(0);
-          }
-#pragma unroll
-          for (int q = 0; q < RT; ++q) {
-            const int c = tx + 16 * q;
-            const T v = S[(k0 + j) * ld + c];
-            bv[q] = ((c >= kk) & (c < n)) ? v : T(0);
-          }
-#pragma unroll
-          for (int a = 0; a < RT; ++a)
-#pragma unroll
-            for (int q = 0; q < RT; ++q) acc[a][q] += av[a] * bv[q];
-        }
-#pragma unroll
-        for (int a = 0; a < RT; ++a)
-#pragma unroll
-          for (int q = 0; q < RT; ++q) S[(ty + 16 * a) * ld + tx + 16 * q] -= acc[a][q];
-      }
-    }
+    if (k0 + nb < n) lds_panel_update<T, RT>(S, n, ld, k0, nb);
     __syncthreads();
   }
 }
