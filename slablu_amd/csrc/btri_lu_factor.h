@@ -141,10 +141,15 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor(int bs, ChainSet<T> c0, 
 // distinct banks.
 __host__ __device__ inline int lu_mfma_ldx(int bs) { return bs <= 16 ? 16 : ((bs - 16 + 31) / 32) * 32 + 16; }
 
+// inverses of the 16 x 16 diagonal blocks of both triangular factors, rows
+// padded to 17
+__host__ __device__ inline int lu_mfma_dinv(int bs) { return 2 * tile_rt(bs) * 16 * 17; }
+
 template <typename T>
 size_t lu_mfma_lds(int bs) {
   const size_t bp = 16 * (size_t)tile_rt(bs);
-  return sizeof(T) * (bp * (bp + 1) + 2 * (size_t)bs * lu_mfma_ldx(bs)) + 2 * sizeof(int) * bs;
+  return sizeof(T) * (bp * (bp + 1) + 2 * (size_t)bs * lu_mfma_ldx(bs) + lu_mfma_dinv(bs) + 64) +
+         2 * sizeof(int) * bs;
 }
 
 template <typename T>
@@ -314,126 +319,186 @@ __device__ __attribute__((always_inline)) void schur_mfma(T* S, int ld, const T*
     const int r = j * 16 + i16;
     aoff[j] = r < bm1 ? r : bm1;
   }
+  const int klast = ((bs - 1) >> 2) << 2;
   for (int ct = wave; ct < RT; ct += 4) {
     acc_t acc[RT];
 #pragma unroll
     for (int j = 0; j < RT; ++j) acc[j] = acc_t{0, 0, 0, 0};
     const int col = ct * 16 + i16;
     const int colc = col < bm1 ? col : bm1;
-#pragma unroll 2
-    for (int k0 = 0; k0 < bs; k0 += 4) {
+    // K step k0's fragments (K past bs as zeros)
+    auto fetch = [&](int k0, T (&av)[RT], T& b) __attribute__((always_inline)) {
       const int k = k0 + kq;
       const bool kin = k < bs;
       const int kr = (kin ? k : bm1) * ldx;
-      T av[RT];
       const T bv = XU[kr + colc];
+      b = kin ? bv : T(0);
 #pragma unroll
-      for (int j = 0; j < RT; ++j) av[j] = XL[kr + aoff[j]];
-      const T b = kin ? bv : T(0);
-#pragma unroll
-      for (int j = 0; j < RT; ++j) acc[j] = Mfma<T>::mma(kin ? av[j] : T(0), b, acc[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < RT; ++j) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = j * 16 + Mfma<T>::row(lane, q);
-        if ((r < bs) & (col < bs)) S[r * ld + col] -= acc[j][q];
+      for (int j = 0; j < RT; ++j) {
+        const T v = XL[kr + aoff[j]];
+        av[j] = kin ? v : T(0);
       }
+    };
+    // the next step's fragments load while this step's MFMAs run
+    T an[RT], bn;
+    fetch(0, an, bn);
+    for (int k0 = 0; k0 < bs; k0 += 4) {
+      T av[RT];
+#pragma unroll
+      for (int j = 0; j < RT; ++j) av[j] = an[j];
+      const T b = bn;
+      fetch(k0 + 4 < klast ? k0 + 4 : klast, an, bn);
+#pragma unroll
+      for (int j = 0; j < RT; ++j) acc[j] = Mfma<T>::mma(av[j], b, acc[j]);
     }
+    // rows and columns past bs land in the padding of S (never read)
+#pragma unroll
+    for (int j = 0; j < RT; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) S[(j * 16 + Mfma<T>::row(lane, q)) * ld + col] -= acc[j][q];
   }
   __syncthreads();
 }
 
-// One 16-row diagonal block (rows d0 .. d0+tb-1) of M X = X for column c,
-// by substitution in registers (M lower, read through `at`; rows past n-1
-// are read clamped and never stored).
-template <typename T, bool UNIT, typename At>
-__device__ inline void diag_block_solve(At at, T* X, int ldx, int d0, int tb, int nm1, int c) {
-  T x[16];
+// Inverses of the 16 x 16 diagonal blocks of M1 (unit lower, through at1)
+// and M2 (lower, through at2), both n x n: Dinv[(m * RT + kb) * 272 + r * 17
+// + c], one thread per (matrix, block, column) solving for a unit vector by
+// substitution.  Entries past the block's last row or column are zero.
+template <typename T, int RT, typename A1, typename A2>
+__device__ __attribute__((always_inline)) void diag_block_inverses(A1 at1, A2 at2, int n, T* Dinv) {
+  const int nm1 = n - 1;
+  for (int task = threadIdx.x; task < 2 * RT * 16; task += 256) {
+    const bool second = task >= RT * 16;
+    const int u = second ? task - RT * 16 : task;
+    const int kb = u >> 4, c = u & 15;
+    const int d0 = kb * 16;
+    const int tb = (n - d0) < 16 ? (n - d0) : 16;
+    T x[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) x[j] = X[(d0 + (j < tb ? j : tb - 1)) * ldx + c];
+    for (int j = 0; j < 16; ++j) x[j] = (j == c) ? T(1) : T(0);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int jj = (d0 + j) < nm1 ? (d0 + j) : nm1;
-    const T xj = UNIT ? x[j] : x[j] / at(jj, jj);
-    x[j] = xj;
+    for (int j = 0; j < 16; ++j) {
+      const int jj = (d0 + j) < nm1 ? (d0 + j) : nm1;
+      const T xj = second ? x[j] / at2(jj, jj) : x[j];
+      x[j] = xj;
 #pragma unroll
-    for (int r = j + 1; r < 16; ++r) x[r] -= at((d0 + r) < nm1 ? (d0 + r) : nm1, jj) * xj;
+      for (int r = j + 1; r < 16; ++r) {
+        const int rr = (d0 + r) < nm1 ? (d0 + r) : nm1;
+        x[r] -= (second ? at2(rr, jj) : at1(rr, jj)) * xj;
+      }
+    }
+    T* D = Dinv + (second ? RT + kb : kb) * 272;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) D[r * 17 + c] = ((r < tb) & (c < tb)) ? x[r] : T(0);
   }
-#pragma unroll
-  for (int j = 0; j < 16; ++j)
-    if (j < tb) X[(d0 + j) * ldx + c] = x[j];
+  __syncthreads();
 }
 
-// Block row d0 .. d0+15 of X -= M[d0.., 0..d0) X[0..d0, :] for column
-// tiles ct0, ct0 + 2, ct0 + 4, ct0 + 6 (those below RT): one M fragment
-// and four X fragments per K step feed four independent MFMAs.
+// X <- M^{-1} X on column tiles ct0, ct0 + 2, ct0 + 4, ct0 + 6 (those below
+// RT) of the n x n X (row stride ldx), M lower with diagonal-block inverses
+// D (RT blocks of 16 x 17): per 16-row block, Y = X_k - M[k, 0:k) X[0:k)
+// by MFMA (one M fragment, four X fragments, four independent products per
+// K step), Y back into X's block rows, then X_k = Dinv_k Y by MFMA.  Only
+// this wavefront's columns are read or written, so no workgroup barrier
+// is needed.
 template <typename T, int RT, typename At>
-__device__ __attribute__((always_inline)) void block_row_update(At at, T* X, int ldx, int n, int d0, int ct0) {
+__device__ __attribute__((always_inline)) void tri_solve_cols_mfma(At at, const T* D, T* X, int ldx, int n,
+                                                                   int ct0, T* junk) {
   typedef typename Mfma<T>::acc_t acc_t;
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, kq = lane >> 4;
   const int nm1 = n - 1;
-  const int rowc = (d0 + i16) < nm1 ? (d0 + i16) : nm1;
-  int coff[4];
+  int col[4], colc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int col = (ct0 + 2 * q) * 16 + i16;
-    coff[q] = col < nm1 ? col : nm1;
+    col[q] = (ct0 + 2 * q) * 16 + i16;
+    colc[q] = col[q] < nm1 ? col[q] : nm1;
   }
-  acc_t acc[4];
+  for (int kb = 0; kb < RT; ++kb) {
+    const int d0 = kb << 4;
+    const int rowc = (d0 + i16) < nm1 ? (d0 + i16) : nm1;
+    acc_t acc[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
-  // d0 is a multiple of 16: every K step is in range
-#pragma unroll 4
-  for (int k0 = 0; k0 < d0; k0 += 4) {
-    const int k = k0 + kq;
-    const T a = at(rowc, k);
-    T bv[4];
+    for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
+    // d0 is a multiple of 16: 16 K values per round, all 20 fragments
+    // loaded before the round's 16 MFMAs
+    for (int k0 = 0; k0 < d0; k0 += 16) {
+      T a[4], bv[4][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bv[q] = X[k * ldx + coff[q]];
+      for (int ks = 0; ks < 4; ++ks) {
+        const int k = k0 + 4 * ks + kq;
+        a[ks] = at(rowc, k);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = Mfma<T>::mma(a, bv[q], acc[q]);
-  }
+        for (int q = 0; q < 4; ++q) bv[ks][q] = X[k * ldx + colc[q]];
+      }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int col = (ct0 + 2 * q) * 16 + i16;
-    if ((ct0 + 2 * q < RT) & (col < n)) {
+      for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = Mfma<T>::mma(a[ks], bv[ks][q], acc[q]);
+    }
+    // Y = X_k - acc into the block rows; entries past n - 1 go to a junk
+    // slot, so the 16 read-modify-writes issue back to back
+    T* dst[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) {
         const int r = d0 + Mfma<T>::row(lane, qq);
-        if (r < n) X[r * ldx + col] -= acc[q][qq];
+        dst[q][qq] = ((r < n) & (col[q] < n)) ? X + r * ldx + col[q] : junk;
       }
     }
+    {
+      T y[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) y[q][qq] = *dst[q][qq];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) *dst[q][qq] = y[q][qq] - acc[q][qq];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // X_k = Dinv_k Y (Y rows past n - 1 enter as zeros)
+    const T* Dk = D + kb * 272;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int k = 4 * ks + kq;
+      const bool kin = d0 + k < n;
+      const int kr = (kin ? d0 + k : nm1) * ldx;
+      const T a = Dk[i16 * 17 + k];
+      T bv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const T v = X[kr + colc[q]];
+        bv[q] = kin ? v : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = Mfma<T>::mma(a, bv[q], acc[q]);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) *dst[q][qq] = acc[q][qq];
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
 // X1 <- M1^{-1} X1 (M1 unit lower, through at1) and X2 <- M2^{-1} X2 (M2
 // lower, through at2), n x n each with row stride ldx, RT = ceil(n/16) <= 8
-// block rows, left-looking: per block row, the MFMA product with the rows
-// already solved (wavefronts 0-1 X1, 2-3 X2, alternate column tiles), then
-// the diagonal block by substitution (wavefronts 0-1 X1, 2-3 X2, a thread
-// per column).
+// tiles: the diagonal-block inverses, then wavefronts 0-1 solve X1 and 2-3
+// X2 on alternate column tiles, each on its own columns only.
 template <typename T, int RT, typename A1, typename A2>
-__device__ __attribute__((always_inline)) void tri_solve2_mfma(A1 at1, A2 at2, int n, T* X1, T* X2, int ldx) {
-  const int tid = threadIdx.x, wave = tid >> 6;
-  const int nm1 = n - 1;
-  for (int kb = 0; kb < RT; ++kb) {
-    const int d0 = kb << 4;
-    const int tb = (n - d0) < 16 ? (n - d0) : 16;
-    if (kb > 0) {
-      if (wave < 2) block_row_update<T, RT>(at1, X1, ldx, n, d0, wave & 1);
-      else block_row_update<T, RT>(at2, X2, ldx, n, d0, wave & 1);
-      __syncthreads();
-    }
-    const int c = tid & 127;
-    if (c < n) {
-      if (tid < 128) diag_block_solve<T, true>(at1, X1, ldx, d0, tb, nm1, c);
-      else diag_block_solve<T, false>(at2, X2, ldx, d0, tb, nm1, c);
-    }
-    __syncthreads();
-  }
+__device__ __attribute__((always_inline)) void tri_solve2_mfma(A1 at1, A2 at2, int n, T* X1, T* X2, int ldx,
+                                                               T* Dinv, T* junk) {
+  diag_block_inverses<T, RT>(at1, at2, n, Dinv);
+  const int wave = threadIdx.x >> 6;
+  if (wave < 2) tri_solve_cols_mfma<T, RT>(at1, Dinv, X1, ldx, n, wave & 1, junk + 16 * wave);
+  else tri_solve_cols_mfma<T, RT>(at2, Dinv + RT * 272, X2, ldx, n, wave & 1, junk + 16 * wave);
+  __syncthreads();
 }
 
 template <typename T, int RT>
@@ -445,7 +510,9 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor_mfma(int bs, ChainSet<T>
   const int ldx = lu_mfma_ldx(bs);
   T* XU = S + BP * (BP + 1);              // Ufac of the current block
   T* XL = XU + bs * ldx;                  // Lfac^T of the current block
-  int* piv = reinterpret_cast<int*>(XL + bs * ldx);
+  T* Dinv = XL + bs * ldx;                // diagonal-block inverses of L and U^T
+  T* junk = Dinv + lu_mfma_dinv(bs);      // 64 entries: targets of masked-off stores
+  int* piv = reinterpret_cast<int*>(junk + 64);
   int* Q = piv + bs;
   __shared__ __align__(16) int wsp[6 * 16 + 2 * 16 * (int)(sizeof(T) / sizeof(int))];
   const int ld = BP + 1;
@@ -472,7 +539,7 @@ __global__ __launch_bounds__(256) void k_btri_lu_factor_mfma(int bs, ChainSet<T>
         piv_to_perm(piv, bs, Q);
         gather_couplings<T>(XU, XL, ldx, Ui, Li, Q, bs);
         // Ufac = L^{-1} (P sup_i);  U^T Lfac^T = sub_i^T
-        tri_solve2_mfma<T, RT>(Lat, UTat, bs, XU, XL, ldx);
+        tri_solve2_mfma<T, RT>(Lat, UTat, bs, XU, XL, ldx, Dinv, junk);
         store_couplings<T>(Ui, Li, XU, XL, ldx, bs);
       }
     }

@@ -427,7 +427,7 @@ __device__ inline void panel_slots(const int* piv, int k0, int nb, int* s_slot, 
 }
 
 // S[kk.., kk..] -= S[kk.., k0..k0+nb) S[k0..k0+nb), kk..] with kk = k0 + nb a
-// multiple of 16 (n <= 16RT, row stride ld): wavefront w takes column tiles
+// multiple of 16 (n <= 16RT, S padded to 16RT x 16RT): wavefront w takes column tiles
 // w, w + 4, ... from kk's tile on and every row tile (rows above kk multiply
 // zeros), so each K step reads one fragment of the panel's row block and RT
 // of its column block, then issues RT independent MFMAs.
@@ -459,15 +459,12 @@ __device__ __attribute__((always_inline)) void lds_panel_update(T* S, int n, int
 #pragma unroll
       for (int j = 0; j < RT; ++j) acc[j] = Mfma<T>::mma((kin & (j >= t0)) ? av[j] : T(0), b, acc[j]);
     }
+    // rows above kk subtract zeros; rows and columns past n land in the
+    // padding of S (never read)
 #pragma unroll
     for (int j = 0; j < RT; ++j) {
-      if (j >= t0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = j * 16 + Mfma<T>::row(lane, q);
-          if ((r < n) & (col < n)) S[r * ld + col] -= acc[j][q];
-        }
-      }
+      for (int q = 0; q < 4; ++q) S[(j * 16 + Mfma<T>::row(lane, q)) * ld + col] -= acc[j][q];
     }
   }
 }
@@ -475,15 +472,15 @@ __device__ __attribute__((always_inline)) void lds_panel_update(T* S, int n, int
 // Blocked in-place LU with partial pivoting of the n x n matrix S in LDS
 // (row stride ld >= 16RT + 1, padded to 16RT rows and columns, n <= 16RT).
 // Each 16-column panel is factored by wavefront 0 alone with the panel in
-// registers (lane l holds panel rows l + 64s): DPP argmax pivot search,
-// pivot row and row exchange through LDS, division by the pivot, rank-1
-// update in registers; no workgroup barrier inside the panel.  Then one
-// thread per column outside the panel moves the entries the panel's
-// interchanges touch (slots from panel_slots) and, right of the panel,
-// solves with the unit-lower L11; then every thread updates its register
-// tile of the trailing matrix (multipliers of finished rows and entries
-// of padding rows/columns enter as zeros).  Four barriers per panel.
-// piv[k] is 0-based.  wsp: LDS workspace of 6 * 16 ints + 32 T.
+// registers (lane l holds panel rows l + 64s): rows never move, each keeps
+// its logical position, the DPP argmax ranks (|a|, logical position), the
+// pivot row reaches every lane by readlanes, then division by the pivot and
+// the rank-1 update in registers; no LDS traffic or barrier per column, and
+// the rows go back to LDS in logical order.  Then one thread per column
+// outside the panel moves the entries the panel's interchanges touch (slots
+// from panel_slots) and, right of the panel, solves with the unit-lower L11;
+// then MFMA tiles update the trailing matrix.  Three barriers per panel.
+// piv[k] is 0-based.  wsp: LDS workspace of 6 * 16 ints.
 template <typename T, int RT>
 __device__ __attribute__((always_inline)) void lu_lds_blocked(T* S, int n, int ld, int* piv, int* wsp, int* info, int64_t b) {
   constexpr int NB = 16;
@@ -493,15 +490,16 @@ __device__ __attribute__((always_inline)) void lu_lds_blocked(T* S, int n, int l
   int* s_row = wsp + NB;             // 2 NB
   int* s_src = wsp + 3 * NB;         // 2 NB
   int* s_cnt = wsp + 5 * NB;         // 1
-  T* xch = reinterpret_cast<T*>(wsp + 6 * NB);  // 2 NB
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = (n - k0) < NB ? (n - k0) : NB;
     const int R = n - k0;
     if (wave == 0) {
       T x[RPW][NB];
+      int lp[RPW];  // logical position of each register row (rows stay put)
 #pragma unroll
       for (int s = 0; s < RPW; ++s) {
         const int rr = lane + 64 * s;
+        lp[s] = rr;
         const int rc = rr < R - 1 ? rr : R - 1;
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
@@ -509,80 +507,95 @@ __device__ __attribute__((always_inline)) void lu_lds_blocked(T* S, int n, int l
           x[s][c] = ((rr < R) & (c < nb)) ? v : T(0);
         }
       }
-      static_for<0, NB>([&](auto jc) __attribute__((always_inline)) {
+      int zc = 0;    // panel column (1-based) of the first exactly zero pivot
+      int pivl = 0;  // lane j: the pivot row of column j
+      // one column: a straight line of VALU/SALU work (no branch, no LDS)
+      // when the panel is full, so consecutive columns can overlap
+      auto column = [&](auto jc, auto fullc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
-        if (j < nb) {
-          T best = T(-1);
-          int bi = INT_MAX;
+        constexpr bool full = decltype(fullc)::value;
+        if (full || j < nb) {
+          // candidates: rows at logical positions j..R-1; the winner is the
+          // largest |a|, ties to the first in order; a NaN ranks below every
+          // number and above the non-candidates
+          int p;
+          if constexpr (sizeof(T) == 4) {
+            unsigned long long best = 0ull;
 #pragma unroll
-          for (int s = 0; s < RPW; ++s) {
-            const int rr = lane + 64 * s;
-            const T v = absv(x[s][j]);
-            const bool ok = (rr >= j) & (rr < R) & (v > best);
-            best = ok ? v : best;
-            bi = ok ? rr : bi;
-          }
-          wave_argmax(best, bi);
-          const int p = (bi < j || bi >= R) ? j : bi;
-          if (lane == 0) piv[k0 + j] = k0 + p;
-          const int lp = p & 63, sp = p >> 6;
-          if (lane == lp) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-              T v = x[0][c];
-#pragma unroll
-              for (int s = 1; s < RPW; ++s) v = bit_blend(v, x[s][c], s == sp);
-              xch[c] = v;
+            for (int s = 0; s < RPW; ++s) {
+              const bool cand = (lp[s] >= j) & (lp[s] < R);
+              const float a = absv(x[s][j]);
+              const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;
+              const unsigned long long k = ((unsigned long long)hi << 32) | (0xFFFFFFFFu - (unsigned)lp[s]);
+              best = umax64(best, cand ? k : 0ull);
             }
-          }
-          if (lane == j) {
+            best = wave_max_u64(best);
+            p = (int)(0xFFFFFFFFu - (unsigned)best);
+          } else {
+            T best = T(-1);
+            int bi = INT_MAX;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) xch[NB + c] = x[0][c];
+            for (int s = 0; s < RPW; ++s) {
+              const bool cand = (lp[s] >= j) & (lp[s] < R);
+              const T a = absv(x[s][j]);
+              argmax_merge(best, bi, cand ? ((a == a) ? a : T(-0.5)) : T(-1), lp[s]);
+            }
+            wave_argmax(best, bi);
+            p = bi;
           }
-          __builtin_amdgcn_wave_barrier();
+          // the register row holding logical position p (exactly one)
+          int pl = 0, ps = 0;
+          static_for<0, RPW>([&](auto sc) __attribute__((always_inline)) {
+            constexpr int sl = decltype(sc)::value;
+            const unsigned long long hit = __ballot(lp[sl] == p);
+            ps = hit != 0ull ? sl : ps;
+            pl = hit != 0ull ? (int)__builtin_ctzll(hit) : pl;
+          });
+          pivl = (lane == j) ? k0 + p : pivl;
+          // the pivot row's columns j.. from its lane
           T pr[NB];
 #pragma unroll
-          for (int c = 0; c < NB; ++c) pr[c] = xch[c];
-          if (p != j) {
-            if (lane == lp) {
+          for (int c = j; c < NB; ++c) {
+            T v = x[0][c];
 #pragma unroll
-              for (int c = 0; c < NB; ++c) {
-                const T v = xch[NB + c];
-#pragma unroll
-                for (int s = 0; s < RPW; ++s) x[s][c] = bit_blend(x[s][c], v, s == sp);
-              }
-            }
-            if (lane == j) {
-#pragma unroll
-              for (int c = 0; c < NB; ++c) x[0][c] = pr[c];
-            }
+            for (int s = 1; s < RPW; ++s) v = bit_blend(v, x[s][c], ps == s);
+            pr[c] = rdlane(v, pl);
           }
+          // interchange j <-> p on the logical positions only
+#pragma unroll
+          for (int s = 0; s < RPW; ++s) lp[s] = (lp[s] == p) ? j : ((lp[s] == j) ? p : lp[s]);
           const T pv = pr[j];
           const bool nz = pv != T(0);
-          if (!nz && lane == 0 && info != nullptr && info[b] == 0) info[b] = k0 + j + 1;
+          zc = (!nz & (zc == 0)) ? j + 1 : zc;
           const T dv = nz ? pv : T(1);
 #pragma unroll
           for (int s = 0; s < RPW; ++s) {
-            const int rr = lane + 64 * s;
-            const bool m = (rr > j) & (rr < R);
+            const bool m = (lp[s] > j) & (lp[s] < R);
             const T l = x[s][j] / dv;
             const T lm = m ? l : T(0);
             x[s][j] = m ? l : x[s][j];
 #pragma unroll
             for (int c = j + 1; c < NB; ++c) x[s][c] -= lm * pr[c];
           }
-          __builtin_amdgcn_wave_barrier();
         }
-      });
+      };
+      if (nb == NB)
+        static_for<0, NB>([&](auto jc) __attribute__((always_inline)) { column(jc, std::true_type{}); });
+      else
+        static_for<0, NB>([&](auto jc) __attribute__((always_inline)) { column(jc, std::false_type{}); });
+      if (lane < nb) piv[k0 + lane] = pivl;
+      if ((zc != 0) & (lane == 0) & (info != nullptr))
+        if (info[b] == 0) info[b] = k0 + zc;
+      // every row to its logical position
 #pragma unroll
       for (int s = 0; s < RPW; ++s) {
-        const int rr = lane + 64 * s;
-        if (rr < R) {
+        if (lp[s] < R) {
 #pragma unroll
           for (int c = 0; c < NB; ++c)
-            if (c < nb) S[(k0 + rr) * ld + k0 + c] = x[s][c];
+            if (c < nb) S[(k0 + lp[s]) * ld + k0 + c] = x[s][c];
         }
       }
+      __builtin_amdgcn_wave_barrier();
       panel_slots(piv, k0, nb, s_slot, s_row, s_src, s_cnt);
     }
     __syncthreads();
