@@ -21,6 +21,10 @@ Beside it, in the same JSON line:
   multi_rhs     132 x 132 leaves, p=16 (N = 3,906,672), 64 right-hand sides:
                 FP64 factor + graph-replayed 64-column solve (autotuned sweep
                 tiles), and the mixed + refined form, error against FP64;
+  hbs           the mixed path with the slab Schur complements recovered as
+                HBS matrices from random samples (rank by the reference's
+                rule) instead of formed exactly: T formation time against
+                the exact one, compressed size, refined error against FP64;
   stages        per-stage times with their algorithmic work (fixed formulas)
                 and the fraction of the MI355X peak they reach;
   rem_factorization  the rem-chain factorization re-run on this build's
@@ -67,7 +71,8 @@ def algorithmic_work(p, n0, n1, buf, Npan):
     )
 
 
-def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.0, pde="waveguide", center=SRC):
+def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.0, pde="waveguide", center=SRC,
+                 hbs_rank=None):
     import torch
     from slablu_amd.src_disc import pdo, built_in_funcs as bif
     from slablu_amd.src_solver.slablu_solver import Domain_Solver
@@ -77,7 +82,7 @@ def make_problem(n, p, ppw, precision, periodic=False, domain="square", box_x=1.
     op, zmap, _ = pdo.get_param_map_and_pdo(domain, bfield, kh)
     a = 1 / (2 * (n / (p - 2)))
     dom = Domain_Solver(torch.tensor([[0, box_x], [0, 1.0]]), op, kh, a, p=p, buf_constant=1.0,
-                        precision=precision, periodic_bc=periodic)
+                        precision=precision, periodic_bc=periodic, hbs_rank=hbs_rank)
     uu_dir = lambda xx: bif.uu_dir_func_greens(zmap(torch.as_tensor(xx)), kh, center=center)
     return dom, uu_dir, kh
 
@@ -173,6 +178,28 @@ def measured_traffic(name):
         return None
     with open(path) as fh:
         return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def hbs_leg(n, p, ppw, extra, sol_fp64, tform_exact_ms):
+    """The mixed path with the slab Schur complements recovered as HBS
+    matrices (rank by the reference's rule) instead of formed exactly: one
+    warm-up and two timed factor + solve steps, the T formation stage, the
+    compressed size, and the refined solution against the exact FP64 one."""
+    import torch
+    dom, uu, _ = make_problem(n, p, ppw, "mixed", hbs_rank="auto", **extra)
+    r = run_path(dom, uu, 2, 1)
+    h = dom.Tslab_hbs
+    err = (torch.linalg.norm(r["sol"] - sol_fp64) / torch.linalg.norm(sol_fp64)).item() if sol_fp64 is not None \
+        else None
+    out = dict(rank=dom.hbs_rank_used(), leaf_blocks="%d x %d" % (h.m, h.bs), samples_per_side=h.bs + h.k,
+               hbs_matrices=h.b, ms_per_step=1e3 * r["t_step"], factor_ms=1e3 * r["t_factor"],
+               solve_ms=1e3 * r["t_solve"], tform_ms=r["stages"].get("toc_Tform"), tform_exact_ms=tform_exact_ms,
+               compressed_bytes=h.nbytes(), dense_bytes=h.b * h.n * h.n * 8,
+               refine_steps=len(dom.refine_history), refine_converged=bool(dom.refine_converged),
+               err_vs_fp64=err)
+    del dom
+    torch.cuda.empty_cache()
+    return out
 
 
 def adjoint_leg(reps=2):
@@ -443,7 +470,7 @@ def main():
         accuracy_ok = accuracy_ok and err <= 1e-8
         del dom64
         torch.cuda.empty_cache()
-    cfg2 = adj = mrhs = None
+    cfg2 = adj = mrhs = hbs = None
     if not args.no_extras and not args.no_fp64:
         print("bench: constant-coefficient fp64", file=sys.stderr, flush=True)
         dom2, uu2, _ = make_problem(n, p, ppw, "fp64", pde="constant")
@@ -453,6 +480,8 @@ def main():
                     solve_ms=1e3 * r2["t_solve"])
         del dom2
         torch.cuda.empty_cache()
+        print("bench: hbs-compressed T_slab", file=sys.stderr, flush=True)
+        hbs = hbs_leg(n, p, ppw, extra, r64["sol"] if fp64 else None, mx["stages"].get("toc_Tform"))
         print("bench: adjoint", file=sys.stderr, flush=True)
         adj = adjoint_leg()
         print("bench: multi-rhs", file=sys.stderr, flush=True)
@@ -481,7 +510,7 @@ def main():
         "factor_ms": 1e3 * mx["t_factor"], "solve_ms": 1e3 * mx["t_solve"],
         "refinement": refine, "accuracy_ok": accuracy_ok, "stages": stages,
         "fp64": fp64, "speedup_vs_fp64": (fp64["ms_per_step"] / (1e3 * t_step)) if fp64 else None,
-        "cfg2_fp64": cfg2, "adjoint": adj, "multi_rhs": mrhs, "rem_factorization": remf,
+        "cfg2_fp64": cfg2, "adjoint": adj, "multi_rhs": mrhs, "hbs": hbs, "rem_factorization": remf,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

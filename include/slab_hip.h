@@ -65,6 +65,14 @@ int slab_getrs(int dtype, int batch, int n, int nrhs, int trans,
                const void* LU, int64_t lda, int64_t lu_bs, const int* ipiv, int64_t p_bs,
                void* B, int64_t ldb, int64_t b_bs, void* stream);
 
+/* Batched orthonormal basis: Q (m x k, row-major) with orthonormal columns
+ * spanning the columns of each A (m x k, m >= k, k <= 256; A is destroyed)
+ * by Householder QR, k columns even when A is rank-deficient.  Replaces the
+ * torch.linalg.qr calls of the HBS reconstruction
+ * (src_solver/hbs_reconstruction.py:454-470): the node bases U, V. */
+int slab_orth(int dtype, int batch, int m, int k, void* A, int64_t lda, int64_t a_bs, void* Q, int64_t ldq,
+              int64_t q_bs, void* stream);
+
 /* Batched inverse X = A^{-1} (n x n, out of place) from the factors of
  * slab_getrf: both triangles inverted by recursive doubling of their
  * diagonal blocks, then X = U^{-1} L^{-1} P.  Replaces torch.linalg.lu_factor

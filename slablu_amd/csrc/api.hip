@@ -165,6 +165,18 @@ int slab_getri(int dtype, int batch, int n, const void* LU, int64_t lda, int64_t
   return -1;
 }
 
+int slab_orth(int dtype, int batch, int m, int k, void* A, int64_t lda, int64_t a_bs, void* Q, int64_t ldq,
+              int64_t q_bs, void* stream) {
+  if (batch < 0 || k < 0 || m < k || k > 256) return -1;
+  if (dtype == SLAB_F64)
+    return status(house_orth<double>(batch, m, k, view<double>(A, lda, 1, a_bs), view<double>(Q, ldq, 1, q_bs),
+                                     S(stream)));
+  if (dtype == SLAB_F32)
+    return status(house_orth<float>(batch, m, k, view<float>(A, lda, 1, a_bs), view<float>(Q, ldq, 1, q_bs),
+                                    S(stream)));
+  return -1;
+}
+
 int slab_getrs_perm(int dtype, int batch, int n, int nrhs, const void* LU, int64_t lda, int64_t lu_bs,
                     const int* perm, int64_t q_bs, void* B, int64_t ldb, int64_t b_bs, void* stream) {
   if (batch < 0 || n < 1 || nrhs < 1 || nrhs > 8) return -1;

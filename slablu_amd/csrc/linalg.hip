@@ -1173,6 +1173,99 @@ hipError_t getrf(int batch, int nrows, int ncols, int nelim, int npiv, View<T> A
   return hipSuccess;
 }
 
+// Sum of v over the 256-thread workgroup, returned to every thread (red:
+// 4 entries of LDS; the caller's next write to red follows a barrier).
+template <typename T>
+__device__ inline T block_sum256(T v, T* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const T t = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return t;
+}
+
+// Orthonormal columns spanning the columns of each m x k matrix A (m >= k,
+// k <= 256), by Householder QR: one workgroup per matrix; per column the
+// norm below the diagonal (workgroup sum), the reflector stored below the
+// diagonal of A (LAPACK's v(0) = 1 scaling, tau in LDS), then thread c
+// applies it to column c > j.  Q = H_0 ... H_{k-1} [I_k; 0] is then
+// accumulated backwards into Q, one thread per column.  A is overwritten.
+// A rank-deficient A still gets k orthonormal columns (a zero column leaves
+// its reflector the identity, so e_j enters Q).
+template <typename T>
+__global__ __launch_bounds__(256) void k_house_orth(int batch, int m, int k, View<T> A, View<T> Q) {
+  __shared__ T tau_s[256];
+  __shared__ T red[4];
+  const int tid = threadIdx.x;
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    for (int j = 0; j < k; ++j) {
+      T s = T(0);
+      for (int i = j + 1 + tid; i < m; i += 256) {
+        const T v = *A.at(b, i, j);
+        s += v * v;
+      }
+      const T sigma = block_sum256(s, red);
+      const T alpha = *A.at(b, j, j);
+      T tau = T(0), beta = alpha, scale = T(1);
+      if (sigma > T(0)) {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
+      }
+      for (int i = j + 1 + tid; i < m; i += 256) *A.at(b, i, j) *= scale;
+      __syncthreads();
+      if (tid == 0) {
+        *A.at(b, j, j) = beta;
+        tau_s[j] = tau;
+      }
+      const int c = tid;
+      if (c > j && c < k && tau != T(0)) {
+        T w = *A.at(b, j, c);
+#pragma unroll 8
+        for (int i = j + 1; i < m; ++i) w += *A.at(b, i, j) * *A.at(b, i, c);
+        w *= tau;
+        *A.at(b, j, c) -= w;
+#pragma unroll 8
+        for (int i = j + 1; i < m; ++i) *A.at(b, i, c) -= *A.at(b, i, j) * w;
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < m * k; e += 256) {
+      const int r = e / k, c = e - r * k;
+      *Q.at(b, r, c) = (r == c) ? T(1) : T(0);
+    }
+    __syncthreads();
+    for (int j = k - 1; j >= 0; --j) {
+      const T tau = tau_s[j];
+      const int c = tid;
+      // columns below j are e_c, which H_j leaves alone
+      if (c >= j && c < k && tau != T(0)) {
+        T w = *Q.at(b, j, c);
+#pragma unroll 8
+        for (int i = j + 1; i < m; ++i) w += *A.at(b, i, j) * *Q.at(b, i, c);
+        w *= tau;
+        *Q.at(b, j, c) -= w;
+#pragma unroll 8
+        for (int i = j + 1; i < m; ++i) *Q.at(b, i, c) -= *A.at(b, i, j) * w;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T>
+hipError_t house_orth(int batch, int m, int k, View<T> A, View<T> Q, hipStream_t s) {
+  if (batch <= 0 || k <= 0) return hipSuccess;
+  if (m < k || k > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_house_orth<T>, dim3(grid_batch(batch)), dim3(256), 0, s, batch, m, k, A, Q);
+  return hipGetLastError();
+}
+
+template hipError_t house_orth<float>(int, int, int, View<float>, View<float>, hipStream_t);
+template hipError_t house_orth<double>(int, int, int, View<double>, View<double>, hipStream_t);
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy(int batch, int rows, int cols, View<T> src, View<T> dst) {
   const int64_t n = (int64_t)rows * cols;

@@ -94,6 +94,10 @@ hipError_t copy_batched(int batch, int rows, int cols, View<T> src, View<T> dst,
 template <typename T>
 hipError_t set_identity(int batch, int n, View<T> dst, hipStream_t s);
 // X(b) = A(b)^{-1} from the factors of getrf (n x n), out of place; ws holds
+// orthonormal columns of each m x k block (Householder, A destroyed)
+template <typename T>
+hipError_t house_orth(int batch, int m, int k, View<T> A, View<T> Q, hipStream_t s);
+
 // getri_ws_bytes(batch, n, sizeof(T)) bytes
 size_t getri_ws_bytes(int batch, int n, size_t elem);
 template <typename T>

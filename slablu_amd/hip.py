@@ -37,6 +37,7 @@ def _load():
         "slab_getrs": ([_int, _int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_piv_to_perm": ([_int, _int, _vp, _i64, _vp, _i64, _vp], _int),
         "slab_getri_ws": ([_int, _int, _int], ctypes.c_size_t),
+        "slab_orth": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_getri": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp], _int),
         "slab_getri_noalloc": ([_int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
         "slab_getrs_perm": ([_int, _int, _int, _int, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp], _int),
@@ -195,6 +196,20 @@ def piv_to_perm(ipiv, n=None):
     _check(LIB.slab_piv_to_perm(b, n, _p(ipiv), ipiv.stride(0), _p(perm), perm.stride(0), _stream()),
            "slab_piv_to_perm")
     return perm
+
+
+def orth(A, Q=None):
+    """Q (batch, m, k) with orthonormal columns spanning the columns of each
+    A (batch, m, k), m >= k, k <= 256, by Householder QR (A is destroyed)."""
+    _require(A.dim() == 3 and A.stride(2) == 1 and A.shape[1] >= A.shape[2] and A.shape[2] <= 256,
+             "orth expects (batch, m, k) row-major blocks with m >= k, k <= 256")
+    b, m, k = A.shape
+    if Q is None:
+        Q = torch.empty(b, m, k, dtype=A.dtype, device=A.device)
+    _require(Q.shape == A.shape and Q.dtype == A.dtype and Q.stride(2) == 1, "orth output layout")
+    _check(LIB.slab_orth(_dt(A), b, m, k, _p(A), A.stride(1), A.stride(0), _p(Q), Q.stride(1), Q.stride(0),
+                         _stream()), "slab_orth")
+    return Q
 
 
 def getri(LU, ipiv, X=None):

@@ -45,6 +45,7 @@ from ..src_disc import hps_multidomain_disc
 from .block_tridiag_separated import SeparatedBlockTridiagonal, separated_period
 from .block_tridiag_solver_dense import Block_Tridiagonal
 from .block_tridiag_solver_structured import HBS_BlockTridiagonal
+from .hbs_reconstruction import HBS_matrix
 
 L_, R_, D_, U_ = 0, 1, 2, 3
 
@@ -75,7 +76,8 @@ def _blockdiag(blocks):
 class Domain_Solver:
     def __init__(self, box_geom, pdo_op, kh, h, p=0, buf_constant=0.5, periodic_bc=False, precision="fp64",
                  device=None, refine_tol=1e-8, refine_maxit=60, refine_method="fgmres", refine_restart=40,
-                 check_pivots="warn", on_unconverged="warn", leaf_memory_budget=None, leaf_chunk=None):
+                 check_pivots="warn", on_unconverged="warn", leaf_memory_budget=None, leaf_chunk=None,
+                 hbs_rank=None):
         if p == 0:
             raise NotImplementedError("finite-difference discretizations are not provided (HPS only, p > 0)")
         if precision not in ("fp64", "mixed"):
@@ -83,6 +85,12 @@ class Domain_Solver:
         if check_pivots not in ("warn", "raise", "ignore") or on_unconverged not in ("warn", "raise", "ignore"):
             raise ValueError("check_pivots / on_unconverged must be 'warn', 'raise' or 'ignore'")
         self.kh = kh
+        # slab Schur complements T_slab: formed exactly (None), or recovered as
+        # HBS matrices of this rank from random samples ("auto": the
+        # reference's rank rule, slablu_solver.py:1215-1229)
+        if hbs_rank is not None and hbs_rank != "auto" and int(hbs_rank) < 1:
+            raise ValueError("hbs_rank must be None, 'auto' or a positive rank")
+        self.hbs_rank = hbs_rank
         self.periodic_bc = periodic_bc
         self.buf_constant = buf_constant
         self.precision = precision
@@ -435,6 +443,12 @@ class Domain_Solver:
         if Np < 2:
             self.Tred_solver = None
             return
+        if self.hbs_rank is not None:
+            Kf = self._Kf_hbs()
+            if getattr(self, "_mark", None) is not None:
+                self._mark("tform")
+            self._interface_from_slabs(Kf)
+            return
         # G = A[rem, X_j] - A[rem, e_j] Y_j, scattered into the padded rem rows
         G = self.cXr - self.cCY
         upad = torch.zeros(Np, (n1 + 1) * B, 2 * L, dtype=dt, device=dev)
@@ -463,6 +477,12 @@ class Domain_Solver:
         del upad, K
         if getattr(self, "_mark", None) is not None:
             self._mark("tform")
+        self._interface_from_slabs(Kf)
+
+    def _interface_from_slabs(self, Kf):
+        """The interface block-tridiagonal system from the slab Schur
+        complements Kf (Np, 2L, 2L) on [L faces; R faces], factored."""
+        Np, L, dev = self.Npan, self.Npan_loc * (self.hps.p - 2), self.device
         Aii = self.interface_blocks()
         KR = Kf[:, L:, L:].to(torch.float64)
         KL = Kf[:, :L, :L].to(torch.float64)
@@ -481,6 +501,63 @@ class Domain_Solver:
             sub = -Kf[1:Np - 1, L:, :L].to(torch.float64)
             self.Tred_solver = HBS_BlockTridiagonal(Np - 1, L)
         self.Tred_solver.build_lowmem(diag, None, sub, sup, device=dev, dtype=torch.float64)
+
+    def hbs_rank_used(self):
+        """Rank of the T_slab compression: 4 x the waves across a slab +
+        1.5 p + 30 for "auto" (slablu_solver.py:1215-1229), else hbs_rank."""
+        if self.hbs_rank == "auto":
+            f = self.hps.p - 2
+            buf, L = self.buf_pans * f, self.Npan_loc * f
+            nwaves_thin = (self.kh / (2 * np.pi)) * (buf / L)
+            return int(4 * nwaves_thin) + int(1.5 * self.hps.p) + 30
+        return int(self.hbs_rank)
+
+    def _slab_schur_apply(self, x, transpose=False):
+        """K_s x (or K_s^T x) for every slab s, K_s = A[X_s, C_s] A_CC,s^{-1}
+        A[C_s, X_s] the slab's Schur complement on its faces:
+        x (Np, 2L, nc) [left faces; right faces] -> (Np, 2L, nc)."""
+        Np, n1, f = self.Npan, self.Npan_loc, self.elim_bs
+        nc = x.shape[-1]
+        xs = x.view(Np, 2, n1, f, nc).permute(0, 2, 1, 3, 4).contiguous()
+        v = self._ACX(xs.view(Np, n1, 2 * f, nc), transpose)
+        _, ue, upad = self._slab_solve(v, transpose)
+        Z = self._AXC(ue, upad, transpose)
+        return Z.permute(0, 2, 1, 3, 4).reshape(Np, 2 * n1 * f, nc)
+
+    def _Kf_hbs(self):
+        """Slab Schur complements recovered as HBS matrices from random
+        samples (slablu_solver.py:1139-1213): K_s applied to
+        blockdiag(Omega, Omega) and K_s^T to blockdiag(Psi, Psi) through the
+        slab solves, the four L x L blocks of every slab reconstructed as one
+        batch of 4 Npan HBS matrices (kept in self.Tslab_hbs), then densified
+        for the interface factorization as the reference's build_lowmem
+        does.  FP64 (Np, 2L, 2L)."""
+        Np, L, dev, dt = self.Npan, self.Npan_loc * (self.hps.p - 2), self.device, self.wdt
+        k = self.hbs_rank_used()
+        hbs = HBS_matrix(4 * Np, L, k)
+        ns = hbs.bs + k
+        if ns > L or hbs.m < 2:
+            raise ValueError("HBS rank %d: %d samples per side and %d leaf blocks for %d interface points "
+                             "(needs at most L samples and two leaf blocks)" % (k, ns, hbs.m, L))
+        g = torch.Generator(device=dev)
+        g.manual_seed(20240613)
+        Omega = torch.rand(L, ns, generator=g, dtype=torch.float64, device=dev)
+        Psi = torch.rand(L, ns, generator=g, dtype=torch.float64, device=dev)
+        X = torch.zeros(Np, 2 * L, 2 * ns, dtype=dt, device=dev)
+        X[:, :L, :ns] = Omega.to(dt)
+        X[:, L:, ns:] = Omega.to(dt)
+        Yf = self._slab_schur_apply(X).double()
+        X[:, :L, :ns] = Psi.to(dt)
+        X[:, L:, ns:] = Psi.to(dt)
+        Zf = self._slab_schur_apply(X, transpose=True).double()
+        del X
+        # blocks LL, LR, RL, RR of every slab: Y = K Omega, Z = K^T Psi
+        Y = torch.stack((Yf[:, :L, :ns], Yf[:, :L, ns:], Yf[:, L:, :ns], Yf[:, L:, ns:]), 1)
+        Z = torch.stack((Zf[:, :L, :ns], Zf[:, L:, :ns], Zf[:, :L, ns:], Zf[:, L:, ns:]), 1)
+        hbs.reconstruct(Omega[None], Psi[None], Y.reshape(4 * Np, L, ns), Z.reshape(4 * Np, L, ns))
+        self.Tslab_hbs = hbs
+        Kd = hbs.todense().view(Np, 2, 2, L, L)
+        return Kd.permute(0, 1, 3, 2, 4).reshape(Np, 2 * L, 2 * L)
 
     # ------------------------------------------------------------------ solve
     def _upad(self, rem, ncols):
