@@ -725,15 +725,11 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
   const int rm1 = R - 1, wm1 = w - 1;
   for (int b = blockIdx.x; b < batch; b += gridDim.x) {
     T x[RP][W];
-    int lp[RP];     // logical position of each row
-    bool dn[RP];    // pivot row already (or a padding row)
-    bool nc[RP];    // never a candidate: padding, or at or past npiv
+    // logical position of each row: the pivot rows of columns 0..j-1 hold
+    // 0..j-1, the candidates of column j sit in j..Rp-1, padding rows at R..
+    int lp[RP];
 #pragma unroll
-    for (int s = 0; s < RP; ++s) {
-      lp[s] = tid + NT * s;
-      dn[s] = lp[s] >= R;
-      nc[s] = lp[s] >= Rp;
-    }
+    for (int s = 0; s < RP; ++s) lp[s] = tid + NT * s;
     constexpr int VN = 16 / sizeof(T);
     typedef T vt __attribute__((ext_vector_type(VN)));
     if (vld) {
@@ -747,7 +743,7 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
         for (int q = 0; q < W / VN; ++q) {
           const vt v = src[q];
 #pragma unroll
-          for (int e = 0; e < VN; ++e) x[s][q * VN + e] = (!dn[s] & (q * VN + e < w)) ? v[e] : T(0);
+          for (int e = 0; e < VN; ++e) x[s][q * VN + e] = ((lp[s] < R) & (q * VN + e < w)) ? v[e] : T(0);
         }
       }
     } else {
@@ -768,31 +764,40 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
   #pragma unroll
           for (int c = 0; c < TW; ++c) {
             const T v = stage[wave][lane][c];
-            x[sl][c0 + c] = (!dn[sl] & (c0 + c < w)) ? v : T(0);
+            x[sl][c0 + c] = ((lp[sl] < R) & (c0 + c < w)) ? v : T(0);
           }
           __syncthreads();
         });
       });
     }
-    int zc = 0;  // panel column (1-based) of the first exactly zero pivot
-    static_for<0, W>([&](auto jc) __attribute__((always_inline)) {
+    int zc = 0;    // panel column (1-based) of the first exactly zero pivot
+    int pivl = 0;  // thread j: the 1-based pivot row of column j
+    // one column.  Columns past w (a narrower last panel) hold zeros: their
+    // pivot is their own row, nothing is updated, and they record no zero
+    // pivot and no ipiv entry, so panels up to 16 wide (and FP64 ones) run
+    // them as straight-line code with no branch but the owner's candidate
+    // row write; wider FP32 panels skip them (the straight line of 32
+    // columns would not fit the registers).
+    constexpr bool kStraight = sizeof(T) == 8 || W <= 16;
+    auto column = [&](auto jc) __attribute__((always_inline)) {
       constexpr int j = decltype(jc)::value;
-      if (j < w) {
+      if (kStraight || j < w) {
         const int par = j & 1;
         unsigned long long best = 0;
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
           const T v = x[s][j];
-          const bool ok = !(dn[s] | nc[s]) & (v == v);
+          const bool ok = (lp[s] >= j) & (lp[s] < Rp) & (v == v);
           best = umax64(best, ok ? pivot_key(v, lp[s]) : 0ull);
         }
         best = wave_max_u64(best);
         if (NW > 1 && lane == 0) red_k[par][wave] = best;
         const int bi = pivot_key_index(best, T(0));
         const bool any = best != 0ull;
+        // the wavefront's best candidate row (logical position bi >= j) to LDS
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
-          if (any & !dn[s] & (lp[s] == bi)) {
+          if (any & (lp[s] == bi)) {
 #pragma unroll
             for (int c = 0; c < W; ++c) cand[par][wave][c] = x[s][c];
           }
@@ -825,7 +830,7 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
           p = valid ? bi : j;
           wv = 0;
         }
-        if (tid == 0) ipiv[(int64_t)b * pbs + k0 + j] = k0 + p + 1;
+        pivl = (tid == j) ? k0 + p + 1 : pivl;
         T pr[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) pr[c] = cand[par][wv][c];
@@ -834,20 +839,18 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
           const int o = lp[s];
-          const bool piv = o == p;
-          lp[s] = piv ? j : ((o == j) ? p : o);
-          dn[s] = dn[s] | piv;
+          lp[s] = (o == p) ? j : ((o == j) ? p : o);
         }
         const T pv = valid ? pr[j] : T(NAN);
         const bool nz = pv != T(0);
-        zc = (!nz & (zc == 0)) ? j + 1 : zc;
+        zc = (!nz & (zc == 0) & (j < w)) ? j + 1 : zc;
         const T dv = nz ? pv : T(1);
         // FP64: one reciprocal per column (as LAPACK's getf2 scales by
         // 1/pivot); FP32: true division, which refinement depends on
         const T rv = sizeof(T) == 8 ? T(1) / dv : T(0);
 #pragma unroll
         for (int s = 0; s < RP; ++s) {
-          const bool m = !dn[s];
+          const bool m = (lp[s] > j) & (lp[s] < R);
           const T l = sizeof(T) == 8 ? x[s][j] * rv : x[s][j] / dv;
           const T lm = m ? l : T(0);
           x[s][j] = m ? l : x[s][j];
@@ -855,7 +858,9 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
           for (int c = j + 1; c < W; ++c) x[s][c] -= lm * pr[c];
         }
       }
-    });
+    };
+    static_for<0, W>(column);
+    if (tid < w) ipiv[(int64_t)b * pbs + k0 + tid] = pivl;
     if ((zc != 0) & (tid == 0) & (info != nullptr))
       if (info[b] == 0) info[b] = k0 + zc;
     if (vst) {
