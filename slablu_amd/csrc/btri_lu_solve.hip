@@ -190,18 +190,45 @@ __global__ __launch_bounds__(256, (INV && NR == 32) ? 2 : 1) void k_btri_lu_solv
           if (e < nn) Mt[slot(e)] = pf[u];
         }
         const T* src = block_src(k, i);
-        // the rest of the block, eight loads in flight per thread
-        for (int e0 = tid + 256 * PF; e0 < nn; e0 += 256 * 8) {
-          T v[8];
+        // the rest of the block, eight loads in flight per thread: for the
+        // FP32 inverse-factor sweeps 16-byte vectors when rows hold whole
+        // vectors (a vector never crosses a row), else single elements
+        constexpr int VN = 16 / sizeof(T);
+        typedef T vt __attribute__((ext_vector_type(VN)));
+        constexpr bool kVecLoad = INV && sizeof(T) == 4;
+        if (kVecLoad && (bs % VN) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+          const vt* s4 = reinterpret_cast<const vt*>(src);
+          const int n4 = nn / VN;
+          for (int q0 = tid + (256 * PF) / VN; q0 < n4; q0 += 256 * 8) {
+            vt v[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int e = e0 + 256 * u;
-            v[u] = src[e < nn ? e : nn - 1];
+            for (int u = 0; u < 8; ++u) {
+              const int q = q0 + 256 * u;
+              v[u] = s4[q < n4 ? q : n4 - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int q = q0 + 256 * u;
+              if (q < n4) {
+                const int o = slot(q * VN);
+#pragma unroll
+                for (int t = 0; t < VN; ++t) Mt[o + (TRANS ? t * ld : t)] = v[u][t];
+              }
+            }
           }
+        } else {
+          for (int e0 = tid + 256 * PF; e0 < nn; e0 += 256 * 8) {
+            T v[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int e = e0 + 256 * u;
-            if (e < nn) Mt[slot(e)] = v[u];
+            for (int u = 0; u < 8; ++u) {
+              const int e = e0 + 256 * u;
+              v[u] = src[e < nn ? e : nn - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = e0 + 256 * u;
+              if (e < nn) Mt[slot(e)] = v[u];
+            }
           }
         }
       }

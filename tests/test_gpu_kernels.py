@@ -368,7 +368,7 @@ def test_btri_factor_solve(dtype, b, m, bs, transpose, twisted):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("b,m,bs", [(3, 5, 20), (2, 1, 16), (2, 2, 37), (2, 11, 100), (3, 4, 100)])
-@pytest.mark.parametrize("nrhs", [1, 3, 32, 70])
+@pytest.mark.parametrize("nrhs", [1, 3, 32, 70, 150])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_btri_twisted_inverse_factors(dtype, b, m, bs, nrhs, transpose):
     """Twisted block LU whose diagonal blocks are applied through their

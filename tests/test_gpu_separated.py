@@ -31,7 +31,7 @@ def _rel(a, b):
 @pytest.mark.parametrize("b,m,bs,d", [(3, 49, 20, 7), (2, 19, 24, 4), (2, 13, 100, 5), (1, 12, 16, 3),
                                       (2, 13, 154, 4)])
 @pytest.mark.parametrize("inv", [False, True])
-@pytest.mark.parametrize("nrhs", [1, 5, 40])
+@pytest.mark.parametrize("nrhs", [1, 5, 40, 140])
 def test_separated_chains_vs_dense(dtype, b, m, bs, d, inv, nrhs):
     from slablu_amd.src_solver.block_tridiag_separated import SeparatedBlockTridiagonal
     print("case separated", dtype, b, m, bs, d, inv, nrhs, flush=True)
