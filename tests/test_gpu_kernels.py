@@ -594,27 +594,44 @@ def test_block_tridiagonal_api(form, twisted):
         assert _rel(xt, torch.linalg.solve(M.transpose(1, 2), f0)) < 1e-11
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 9])
-def test_hbs_block_tridiagonal_api(nb):
-    """Twisted (two-ended) block elimination against a dense solve, for odd,
-    even and degenerate block counts."""
+@pytest.mark.parametrize("order", ["twisted", "separated", "auto"])
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 6, 8, 9, 13])
+def test_hbs_block_tridiagonal_api(nb, order):
+    """Twisted (two-ended) and separated (even blocks, then the odd blocks'
+    Schur complement, recursively) block elimination against a dense solve,
+    plain and transposed, for odd, even and degenerate block counts; the
+    separated order's dependent steps are counted from its levels."""
     from slablu_amd.src_solver.block_tridiag_solver_structured import HBS_BlockTridiagonal
     g = torch.Generator().manual_seed(9 + nb)
-    n = 300
-    d0 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) + 5 * torch.eye(n, dtype=torch.float64)
-    d1 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) * 0.1
-    sub = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.2
-    sup = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.2
-    H = HBS_BlockTridiagonal(nb, n)
+    n = 300 if nb < 13 else 120
+    # well-conditioned blocks (random parts of spectral norm about 2 and 1
+    # around 5 I), so that the tolerance measures the algebra of each order
+    # and not the conditioning of the blocks it inverts
+    s = 1.0 / n ** 0.5
+    d0 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) * s + 5 * torch.eye(n, dtype=torch.float64)
+    d1 = torch.randn(nb, n, n, generator=g, dtype=torch.float64) * 0.1 * s
+    sub = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.5 * s
+    sup = torch.randn(max(nb - 1, 0), n, n, generator=g, dtype=torch.float64) * 0.5 * s
+    H = HBS_BlockTridiagonal(nb, n, order=order)
     H.build_lowmem(d0, d1, sub, sup, device="cuda")
     f = torch.randn(nb * n, 2, generator=g, dtype=torch.float64)
     x = H.solve(f.cuda())
+    xt = H.solve(f.cuda(), transpose=True)
     M = _btri_dense((d0 + d1)[None], sub[None], sup[None])[0]
     torch.cuda.synchronize()
     assert _rel(x, torch.linalg.solve(M, f)) < 1e-11
+    assert _rel(xt, torch.linalg.solve(M.T, f)) < 1e-11
+    assert int((H.info != 0).sum()) == 0
+    # the levels this build ran: one batched round per separated level
+    levels, h = 0, H
+    while getattr(h, "separated", False):
+        levels, h = levels + 1, h.Tsolver
+    assert levels + (h.b // 2 + 1) == HBS_BlockTridiagonal.dependent_steps(nb, order)
+    if order == "separated" and nb >= 2 or order == "auto" and nb >= 6:
+        assert H.separated
 
 
-@pytest.mark.parametrize("nb", [2, 3, 4, 7])
+@pytest.mark.parametrize("nb", [2, 3, 4, 7, 10])
 def test_hbs_block_tridiagonal_cyclic(nb):
     """Cyclic (periodic) interface ring: chain over blocks 0..nb-2 bordered by
     block nb-1, against a dense solve of the ring matrix (for nb = 2 the two
@@ -631,12 +648,15 @@ def test_hbs_block_tridiagonal_cyclic(nb):
         M[j * n:(j + 1) * n, j * n:(j + 1) * n] += diag[j]
         M[j * n:(j + 1) * n, k * n:(k + 1) * n] += sup[j]      # block (j, j+1)
         M[k * n:(k + 1) * n, j * n:(j + 1) * n] += sub[j]      # block (j+1, j)
-    H = HBS_BlockTridiagonal(nb, n, cyclic=True)
-    H.build_lowmem(diag, None, sub, sup, device="cuda")
-    f = torch.randn(nb * n, 3, generator=g, dtype=torch.float64)
-    x = H.solve(f.cuda())
-    torch.cuda.synchronize()
-    assert _rel(x, torch.linalg.solve(M, f)) < 1e-11
+    for order in ("twisted", "separated"):
+        H = HBS_BlockTridiagonal(nb, n, cyclic=True, order=order)
+        H.build_lowmem(diag, None, sub, sup, device="cuda")
+        f = torch.randn(nb * n, 3, generator=g, dtype=torch.float64)
+        x = H.solve(f.cuda())
+        xt = H.solve(f.cuda(), transpose=True)
+        torch.cuda.synchronize()
+        assert _rel(x, torch.linalg.solve(M, f)) < 1e-11
+        assert _rel(xt, torch.linalg.solve(M.T, f)) < 1e-11
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
