@@ -67,15 +67,17 @@ class SeparatedBlockTridiagonal:
         self.twisted = hip.btri_fused(dt, bs)
         if not self.twisted:
             self.inv_factors = False
-        # padded chain: identity diagonal blocks, zero couplings
-        Dp = torch.zeros(b, mp, bs, bs, dtype=dt, device=dev)
-        Dp[:, :m] = diag
-        if mp > m:
-            Dp[:, m:] = torch.eye(bs, dtype=dt, device=dev)
-        Lp = torch.zeros(b, mp, bs, bs, dtype=dt, device=dev)       # Lp[i] = block (i+1, i)
-        Up = torch.zeros(b, mp, bs, bs, dtype=dt, device=dev)       # Up[i] = block (i, i+1)
-        Lp[:, :m - 1] = sub
-        Up[:, :m - 1] = sup
+        # padded chain: identity diagonal blocks, zero couplings (the last
+        # coupling slot of every chain is always padding)
+        if mp > m or not diag.is_contiguous():
+            Dp = torch.empty(b, mp, bs, bs, dtype=dt, device=dev)
+            Dp[:, :m] = diag
+            if mp > m:
+                Dp[:, m:] = torch.eye(bs, dtype=dt, device=dev)
+        else:
+            Dp = diag                                                # only read below
+        Lp = torch.nn.functional.pad(sub, (0, 0, 0, 0, 0, mp - m + 1))   # Lp[i] = block (i+1, i)
+        Up = torch.nn.functional.pad(sup, (0, 0, 0, 0, 0, mp - m + 1))   # Up[i] = block (i, i+1)
         nb = b * P
         Dv, Lv, Uv = (t.view(nb, d, bs, bs) for t in (Dp, Lp, Up))
         # pieces: blocks 0..l-1 of every period
@@ -91,45 +93,42 @@ class SeparatedBlockTridiagonal:
             # products too
             self.plinv = torch.empty_like(self.pd)
             hip.btri_invert_diag(self.pd, self.pipiv, self.plinv)
-        # couplings of the separator of period g (block l) to its pieces
-        self.C_last = Lv[:, l - 1].contiguous()   # (sep g, last block of piece g)
-        self.E_last = Uv[:, l - 1].contiguous()   # (last block of piece g, sep g)
-        self.C_next = Uv[:, l].contiguous()       # (sep g, first block of piece g+1)
-        self.E_next = Lv[:, l].contiguous()       # (first block of piece g+1, sep g)
+        # couplings of the separator of period g (block l) to its pieces:
+        # strided views of the padded couplings (one block per period)
+        self.C_last = Lv[:, l - 1]                # (sep g, last block of piece g)
+        self.E_last = Uv[:, l - 1]                # (last block of piece g, sep g)
+        self.C_next = Uv[:, l]                    # (sep g, first block of piece g+1)
+        self.E_next = Lv[:, l]                    # (first block of piece g+1, sep g)
         # spikes: V[g] = piece_g^{-1} [E_prev | E_last] (E_prev: the coupling of
-        # the piece's first block to the separator before it)
+        # the piece's first block to the separator before it).  Separators of
+        # a chain never couple across chains: E_next of a chain's last period
+        # is a padding zero, so the first piece of the next chain gets none
         V = torch.zeros(nb, l, bs, 2, bs, dtype=dt, device=dev)
         V[1:, 0, :, 0] = self.E_next[:-1]
         V[:, l - 1, :, 1] = self.E_last
-        # separators of a chain never couple across chains: the previous
-        # separator of each chain's first piece does not exist
-        V.view(b, P, l, bs, 2, bs)[:, 0, :, :, 0] = 0
         self.V = V.view(nb, l * bs, 2 * bs)
         self._piece_solve(self.V, False, 0, nb)
         V5 = self.V.view(nb, l, bs, 2, bs)
-        # separator Schur complement, block tridiagonal over the P separators
+        # separator Schur complement, block tridiagonal over the P separators.
+        # The products run flat over all b P periods: the one pairing a
+        # chain's last period with the next chain's first is zero (that
+        # C_next is padding, and the next chain's first piece has no E_prev
+        # spike), and its slot is dropped
         T = Dv[:, l].clone()                                             # (nb, bs, bs)
         hip.gemm(self.C_last, V5[:, l - 1, :, 1], T, alpha=-1.0, beta=1.0)
-        Tn = T.view(b, P, bs, bs)
-        Cn = self.C_next.view(b, P, bs, bs)
-        Vn = V5.view(b, P, l, bs, 2, bs)
         if P > 1:
-            tmp = torch.empty(b, P - 1, bs, bs, dtype=dt, device=dev)
             # diagonal: minus (sep g -> first block of piece g+1) V^prev of piece g+1
-            hip.gemm(Cn[:, :P - 1].reshape(-1, bs, bs), Vn[:, 1:, 0, :, 0].reshape(-1, bs, bs), tmp.view(-1, bs, bs))
-            Tn[:, :P - 1] -= tmp
-            tsup = torch.empty(b, P - 1, bs, bs, dtype=dt, device=dev)
-            tsub = torch.empty(b, P - 1, bs, bs, dtype=dt, device=dev)
+            hip.gemm(self.C_next[:nb - 1], V5[1:, 0, :, 0], T[:nb - 1], alpha=-1.0, beta=1.0)
+            tsup = torch.empty(b, P, bs, bs, dtype=dt, device=dev)
+            tsub = torch.empty(b, P, bs, bs, dtype=dt, device=dev)
             # T(g, g+1) = -C_next[g] V^last_{g+1}[first block]
-            hip.gemm(Cn[:, :P - 1].reshape(-1, bs, bs), Vn[:, 1:, 0, :, 1].reshape(-1, bs, bs),
-                     tsup.view(-1, bs, bs), alpha=-1.0)
+            hip.gemm(self.C_next[:nb - 1], V5[1:, 0, :, 1], tsup.view(nb, bs, bs)[:nb - 1], alpha=-1.0)
             # T(g+1, g) = -C_last[g+1] V^prev_{g+1}[last block]
-            Cl = self.C_last.view(b, P, bs, bs)
-            hip.gemm(Cl[:, 1:].reshape(-1, bs, bs), Vn[:, 1:, l - 1, :, 0].reshape(-1, bs, bs),
-                     tsub.view(-1, bs, bs), alpha=-1.0)
+            hip.gemm(self.C_last[1:], V5[1:, l - 1, :, 0], tsub.view(nb, bs, bs)[:nb - 1], alpha=-1.0)
+            tsup, tsub = tsup[:, :P - 1], tsub[:, :P - 1]
         else:
             tsup = tsub = torch.zeros(b, 1, bs, bs, dtype=dt, device=dev)
-        self.sd, self.sl, self.su = Tn.contiguous(), tsub.contiguous(), tsup.contiguous()
+        self.sd, self.sl, self.su = T.view(b, P, bs, bs), tsub, tsup
         self.sipiv = torch.zeros(b, P, bs, dtype=torch.int32, device=dev)
         self.sinfo = torch.zeros(b, dtype=torch.int32, device=dev)
         hip.btri_factor(self.sd, self.sl, self.su, self.sipiv, info=self.sinfo, twisted=self.twisted)
