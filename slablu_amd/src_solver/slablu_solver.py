@@ -527,7 +527,7 @@ class Domain_Solver:
         nc = x.shape[-1]
         xs = x.view(Np, 2, n1, f, nc).permute(0, 2, 1, 3, 4).contiguous()
         v = self._ACX(xs.view(Np, n1, 2 * f, nc), transpose)
-        _, ue, upad = self._slab_solve(v, transpose)
+        _, ue, upad = self._slab_solve(v, transpose, want_sol=False)
         Z = self._AXC(ue, upad, transpose)
         return Z.permute(0, 2, 1, 3, 4).reshape(Np, 2 * n1 * f, nc)
 
@@ -567,35 +567,32 @@ class Domain_Solver:
         return Kd.permute(0, 1, 3, 2, 4).reshape(Np, 2 * L, 2 * L)
 
     # ------------------------------------------------------------------ solve
-    def _upad(self, rem, ncols):
-        Np, n1, B = rem.shape[0], self.Npan_loc, self.rem_bs
-        upad = torch.zeros(Np, (n1 + 1) * B, ncols, dtype=rem.dtype, device=rem.device)
-        upad[:, B:n1 * B] = rem
-        return upad
-
     def _u_local_all(self, upad):
         Np, n1, B = upad.shape[0], self.Npan_loc, self.rem_bs
         nc = upad.shape[-1]
         return torch.as_strided(upad, (Np, n1, 2 * B, nc), (upad.stride(0), B * nc, nc, 1),
                                 upad.storage_offset()).reshape(Np * n1, 2 * B, nc)
 
-    def _scatter_rem(self, CT):
+    def _scatter_rem(self, CT, out=None):
         """Sum per-leaf-row [bottom; top] rem contributions (Np*n1, 2B, nc)
-        into the rem rows of each slab -> (Np, (n1-1)*B, nc)."""
+        into the rem rows of each slab -> (Np, (n1-1)*B, nc): rem block j
+        (between leaf rows j and j+1) gets the top part of leaf row j+1 and
+        the bottom part of leaf row j; written to out if given."""
         n1, B = self.Npan_loc, self.rem_bs
         Np = CT.shape[0] // n1
         nc = CT.shape[-1]
         C5 = CT.view(Np, n1, 2, B, nc)
-        g = torch.zeros(Np, n1 + 1, B, nc, dtype=CT.dtype, device=CT.device)
-        g[:, :n1] += C5[:, :, 0]
-        g[:, 1:] += C5[:, :, 1]
-        return g[:, 1:n1].reshape(Np, (n1 - 1) * B, nc)
+        if out is None:
+            return torch.add(C5[:, 1:, 0], C5[:, :-1, 1]).view(Np, (n1 - 1) * B, nc)
+        torch.add(C5[:, 1:, 0], C5[:, :-1, 1], out=out.view(Np, n1 - 1, B, nc))
+        return out
 
-    def _slab_solve(self, f_C, transpose=False, s0=0, s1=None):
+    def _slab_solve(self, f_C, transpose=False, s0=0, s1=None, want_sol=True, out=None):
         """A_CC(slab)^{-1} f (transpose: A_CC(slab)^{-T} f) for the slabs
         s0..s1-1 through elim -> rem -> elim (slablu_solver.py:901-943);
         f_C (s1 - s0, |I_C|, nrhs) in the working precision.  Returns (sol,
-        elim part (rows, m f, nrhs), padded rem part (slabs, (n1+1) B, nrhs)).
+        elim part (rows, m f, nrhs), padded rem part (slabs, (n1+1) B, nrhs));
+        sol is None without want_sol, and written to out if given.
         Plain: t = A_ee^{-1} f_e, u = Ared^{-1}(f_r - A_re t),
         x_e = t - (A_ee^{-1} A_er) u.  Transposed: t = A_ee^{-T} f_e,
         u = Ared^{-T}(f_r - A_er^T t), x_e = A_ee^{-T}(f_e - A_re^T u)."""
@@ -607,11 +604,18 @@ class Domain_Solver:
         nc = f_C.shape[-1]
         fe = f_C[:, :ne].reshape(Np * n1, m * f, nc).contiguous()
         fr = f_C[:, ne:]
-        t = self.Aloc_solver.lusolve_sweep(fe.clone(), b_start=s0 * n1, b_end=s1 * n1, transpose=transpose)
+        # the elim sweep runs in place: on fe itself when fe is already a
+        # private copy of f_C's rows (and the plain solve needs it no more)
+        private = fe.untyped_storage().data_ptr() != f_C.untyped_storage().data_ptr()
+        t = self.Aloc_solver.lusolve_sweep(fe if private and not transpose else fe.clone(), b_start=s0 * n1,
+                                           b_end=s1 * n1, transpose=transpose)
         CT = torch.empty(Np * n1, 2 * B, nc, dtype=t.dtype, device=t.device)
         hip.gemm(self.cR[rows].transpose(1, 2) if transpose else self.cC[rows], t, CT)
-        g = fr - self._scatter_rem(CT)
-        upad = self._upad(g, nc)
+        # padded rem right-hand side f_r - (scattered couplings), zero blocks
+        # at both ends
+        upad = torch.zeros(Np, (n1 + 1) * B, nc, dtype=f_C.dtype, device=f_C.device)
+        g = self._scatter_rem(CT, out=upad[:, B:n1 * B])
+        torch.sub(fr, g, out=g)
         self.Ared_solver.lusolve_sweep(upad[:, B:n1 * B], b_start=s0, b_end=s1, transpose=transpose)
         ul = self._u_local_all(upad)
         if transpose:
@@ -619,11 +623,13 @@ class Domain_Solver:
             hip.gemm(self.cC[rows].transpose(1, 2), ul, ue, alpha=-1.0, beta=1.0)
             self.Aloc_solver.lusolve_sweep(ue, b_start=s0 * n1, b_end=s1 * n1, transpose=True)
         else:
-            ue = t.clone()
+            ue = t
             hip.gemm(self.cW[rows], ul, ue, alpha=-1.0, beta=1.0)
-        sol = torch.empty_like(f_C)
-        sol[:, :ne] = ue.view(Np, ne, nc)
-        sol[:, ne:] = upad[:, B:n1 * B]
+        sol = None
+        if want_sol:
+            sol = torch.empty_like(f_C) if out is None else out
+            sol[:, :ne] = ue.view(Np, ne, nc)
+            sol[:, ne:] = upad[:, B:n1 * B]
         return sol, ue, upad
 
     def solve_slab(self, Npan_start, Npan_end, f, transpose=False):
@@ -659,7 +665,7 @@ class Domain_Solver:
         hip.gemm(Ar, xe, r)
         out = torch.empty(Np, len(self.I_C), nc, dtype=x.dtype, device=x.device)
         out[:, :n1 * m * f] = e.view(Np, n1 * m * f, nc)
-        out[:, n1 * m * f:] = self._scatter_rem(r)
+        self._scatter_rem(r, out=out[:, n1 * m * f:])
         return out
 
     def _interface_to_slabs(self, sx, dtype):
@@ -686,6 +692,17 @@ class Domain_Solver:
             return torch.roll(Z[:, :, 1], 1, dims=0).double() + Z[:, :, 0].double()
         return Z[:Np - 1, :, 1].double() + Z[1:, :, 0].double()
 
+    def _interface_rhs(self, fX, Z):
+        """fX - (per-slab contributions Z summed onto the interfaces), in
+        FP64: fX (interfaces, n1, f, nc) in the working precision, Z as
+        _slabs_to_interface takes it."""
+        if self.periodic_bc:
+            return fX.to(torch.float64) - self._slabs_to_interface(Z)
+        Np = self.Npan
+        acc = Z[:Np - 1, :, 1].to(torch.float64, copy=True)
+        acc += Z[1:, :, 0]
+        return torch.sub(fX, acc, out=acc)
+
     def solve_reduced(self, ff, transpose=False):
         """A_CC x = ff on the face unknowns I_Ctot (slablu_solver.py:543-590),
         or A_CC^T x = ff with transpose; ff (|I_Ctot|, nrhs) in the working
@@ -695,16 +712,18 @@ class Domain_Solver:
         L = n1 * f
         nc = ff.shape[-1]
         ffC = ff[:Np * nC].view(Np, nC, nc)
-        yC, ue, upad = self._slab_solve(ffC, transpose)
+        yC, ue, upad = self._slab_solve(ffC, transpose, want_sol=Np == 1)
         if Np == 1:
             return yC.reshape(Np * nC, nc)
         nI = Np if self.periodic_bc else Np - 1
-        ffX = ff[Np * nC:].view(nI, n1, f, nc).to(torch.float64) - self._slabs_to_interface(
-            self._AXC(ue, upad, transpose))
+        ffX = self._interface_rhs(ff[Np * nC:].view(nI, n1, f, nc), self._AXC(ue, upad, transpose))
         solX = self.Tred_solver.solve(ffX.reshape(nI * L, nc), transpose=transpose)
         x = self._interface_to_slabs(solX.view(nI, n1, f, nc), ff.dtype)
-        solC, _, _ = self._slab_solve(ffC - self._ACX(x.view(Np, n1, 2 * f, nc), transpose), transpose)
-        return torch.cat((solC.reshape(Np * nC, nc), solX.to(ff.dtype)), dim=0)
+        out = torch.empty(Np * nC + nI * L, nc, dtype=ff.dtype, device=ff.device)
+        acx = self._ACX(x.view(Np, n1, 2 * f, nc), transpose)
+        self._slab_solve(torch.sub(ffC, acx, out=acx), transpose, out=out[:Np * nC].view(Np, nC, nc))
+        out[Np * nC:] = solX
+        return out
 
     def apply_Tred(self, rhs, transpose=False):
         """Action of the interface Schur complement T = A_XX - A_XC A_CC^{-1}
@@ -717,7 +736,7 @@ class Domain_Solver:
         sx = x64.view(nI, n1, f, nc)
         x = self._interface_to_slabs(sx, self.wdt)
         v = self._ACX(x.view(Np, n1, 2 * f, nc), transpose)
-        _, ue, upad = self._slab_solve(v, transpose)
+        _, ue, upad = self._slab_solve(v, transpose, want_sol=False)
         Z = self._slabs_to_interface(self._AXC(ue, upad, transpose))
         Aii = self.interface_blocks().to(torch.float64)
         if transpose:
@@ -771,7 +790,7 @@ class Domain_Solver:
         solves A_cc^{-1} f_b."""
         ops, lf = self.hps.ops, self.hps.leaf
         ni = ops.ni
-        w = fb.to(lf["dtype"]).contiguous().clone()
+        w = fb.to(lf["dtype"], copy=True).contiguous()
         for s, e, M, ipiv, perm in self.hps.leaf_chunks():
             hip.getrs(M[:, :, :ni], ipiv, w[s:e], perm=perm)
         z = torch.empty(fb.shape[0], ops.nx, fb.shape[-1], dtype=w.dtype, device=w.device)
@@ -870,7 +889,7 @@ class Domain_Solver:
         hps, ops, lf = self.hps, self.hps.ops, self.hps.leaf
         Ic = self._Ic()
         yb, w = self._body_reduce(rI, return_solve=True)
-        ff = rF + yb[Ic].double()
+        ff = torch.add(rF, yb[Ic])
         dC = self.solve_reduced(ff.to(self.wdt)).double()
         dF = torch.zeros(hps.nface, dC.shape[-1], dtype=torch.float64, device=self.device)
         dF[Ic] = dC
@@ -882,10 +901,17 @@ class Domain_Solver:
     def _apply_full(self, dI, dC):
         """A d of the un-condensed system (FP64) for a vector with zero
         Dirichlet data: interior values dI, face values dC on I_Ctot."""
-        x = torch.zeros(self.hps.nface, dC.shape[-1], dtype=torch.float64, device=self.device)
-        x[self._Ic()] = dC
-        rI, rF = self.full_residual(x, dI)
-        return -rI, -rF
+        hps = self.hps
+        Ic = self._Ic()
+        nc = dC.shape[-1]
+        x = torch.zeros(hps.nface, nc, dtype=torch.float64, device=self.device)
+        x[Ic] = dC
+        # full_residual with no load: r_I = -(A d)_I, and the summed outward
+        # fluxes y give (A d)_F = y[I_Ctot]
+        rI, fl = hps.ops.residual(dI, hps.faces_to_boxes(x), hps.coefs)
+        y = torch.zeros(hps.nface, nc, dtype=torch.float64, device=self.device)
+        y.index_add_(0, hps.Iext_d.reshape(-1), fl.reshape(-1, nc))
+        return rI.neg_(), y[Ic]
 
     def _converged(self, c, prev):
         """Stop test on the relative change c of the last step (prev: the
