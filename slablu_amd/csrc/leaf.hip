@@ -40,7 +40,7 @@ __device__ inline void leaf_col_node(int col, int ni, int f, int p, int& j1, int
 // ones, nxc of them, e.g. the 4(p-1) corner-inclusive exterior nodes in
 // Jxreorder order; NULL: the 4(p-2) face nodes in [l, r, d, u] order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nrows, int rows_per_block,
+__global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int row_begin, int nrows, int rows_per_block,
                                                        const double* __restrict__ D, const double* __restrict__ Dsq,
                                                        const double* c11, const double* c22, const double* c12,
                                                        const double* c1, const double* c2, const double* c,
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nr
   const int f = p - 2, ni = f * f, nx = xnodes ? nxc : 4 * f, ncols = ni + nx;
   const int b = blockIdx.x;
   if (b >= nboxes) return;
-  const int r0 = blockIdx.y * rows_per_block;
+  const int r0 = row_begin + blockIdx.y * rows_per_block;
   const int p2 = p * p;
   T* Mb = M + (int64_t)b * mbs;
   // the nodes of this thread's columns, once for all rows of the block
@@ -150,6 +150,84 @@ __global__ __launch_bounds__(256) void k_leaf_assemble(int nboxes, int p, int nr
   }
 }
 
+constexpr int kAsmMaxP2 = 1024;  // p^2 of the LDS copies of D and D^2 (p <= 32)
+
+// The interior rows without a cross term (c12 == NULL), as 16-byte vector
+// stores: each thread owns VN consecutive columns of a row, ncols / VN
+// threads a row, 256 / (ncols / VN) rows at a time; the 1-D Chebyshev
+// tables D and D^2 are read from LDS copies.  The entries are those
+// of k_leaf_assemble (same terms in the same order, FP64, one rounding to
+// T).  Requires ncols % VN == 0, ncols / VN <= 256 and 16-byte aligned rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_assemble_vec(int nboxes, int p, int rows_per_block,
+                                                           const double* __restrict__ D, const double* __restrict__ Dsq,
+                                                           const double* c11, const double* c22, const double* c1,
+                                                           const double* c2, const double* c, T* M, int64_t ldm,
+                                                           int64_t mbs, const int* __restrict__ xnodes, int nxc) {
+#pragma clang fp contract(off)
+  constexpr int VN = 16 / sizeof(T);
+  typedef T vt __attribute__((ext_vector_type(VN)));
+  const int f = p - 2, ni = f * f, nx = xnodes ? nxc : 4 * f, ncols = ni + nx;
+  const int tpr = ncols / VN, rpi = 256 / tpr;
+  const int b = blockIdx.x;
+  const int r0 = blockIdx.y * rows_per_block;
+  const int p2 = p * p;
+  const int t = threadIdx.x;
+  const int q = t % tpr, sub = t / tpr;
+  __shared__ double cf[5][64];
+  __shared__ double sD[kAsmMaxP2], sDsq[kAsmMaxP2];
+  if (t < rows_per_block) {
+    const int r = r0 + t;
+    const int rc = r < ni ? r : ni - 1;
+    const int64_t ci = (int64_t)b * p2 + (1 + rc / f) * p + 1 + rc % f;
+    cf[0][t] = -c11[ci];
+    cf[1][t] = -c22[ci];
+    cf[2][t] = c1 ? c1[ci] : 0.0;
+    cf[3][t] = c2 ? c2[ci] : 0.0;
+    cf[4][t] = c ? c[ci] : 0.0;
+  }
+  for (int e = t; e < p2; e += 256) {
+    sD[e] = D[e];
+    sDsq[e] = Dsq[e];
+  }
+  __syncthreads();
+  if (sub >= rpi || b >= nboxes) return;
+  int jj1[VN], jj2[VN];
+#pragma unroll
+  for (int e = 0; e < VN; ++e) {
+    const int col = q * VN + e;
+    if (xnodes && col >= ni) {
+      const int node = xnodes[col - ni];
+      jj1[e] = node / p;
+      jj2[e] = node % p;
+    } else {
+      leaf_col_node(col, ni, f, p, jj1[e], jj2[e]);
+    }
+  }
+  T* Mb = M + (int64_t)b * mbs;
+  for (int rr = sub; rr < rows_per_block; rr += rpi) {
+    const int r = r0 + rr;
+    if (r >= ni) break;
+    const int i1 = 1 + r / f, i2 = 1 + r % f;
+    const double v11 = cf[0][rr], v22 = cf[1][rr], v1 = cf[2][rr], v2 = cf[3][rr], v0 = cf[4][rr];
+    vt out;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+      const int j1 = jj1[e], j2 = jj2[e];
+      double val = 0.0;
+      if (i1 == j1 || i2 == j2) {
+        val += v11 * (i2 == j2 ? sDsq[i1 * p + j1] : 0.0);
+        val += v22 * (i1 == j1 ? sDsq[i2 * p + j2] : 0.0);
+        if (c1) val += v1 * (-(i2 == j2 ? sD[i1 * p + j1] : 0.0));
+        if (c2) val += v2 * (-(i1 == j1 ? sD[i2 * p + j2] : 0.0));
+        if (c) val += v0 * ((i1 == j1 && i2 == j2) ? 1.0 : 0.0);
+      }
+      out[e] = (T)val;
+    }
+    reinterpret_cast<vt*>(Mb + (int64_t)r * ldm)[q] = out;
+  }
+}
+
 template <typename T>
 hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const double* Dsq, const double* c11,
                          const double* c22, const double* c12, const double* c1, const double* c2, const double* c,
@@ -157,10 +235,24 @@ hipError_t leaf_assemble(int nboxes, int p, int border, const double* D, const d
                          int nxc) {
   const int f = p - 2, ni = f * f, nx = 4 * f;
   const int nrows = ni + (border ? nx : 0);
+  const int ncols = ni + (xnodes ? nxc : nx);
   const int rpb = 32;
-  dim3 grid(nboxes, (nrows + rpb - 1) / rpb);
-  hipLaunchKernelGGL(k_leaf_assemble<T>, grid, dim3(256), 0, s, nboxes, p, nrows, rpb, D, Dsq, c11, c22, c12, c1,
-                     c2, c, Nxtot, M, ldm, mbs, xnodes, nxc);
+  constexpr int VN = 16 / sizeof(T);
+  const bool vec = c12 == nullptr && p * p <= kAsmMaxP2 && ncols % VN == 0 && ncols / VN <= 256 &&
+                   (reinterpret_cast<uintptr_t>(M) & 15) == 0 && ((ldm * (int64_t)sizeof(T)) & 15) == 0 &&
+                   (nboxes == 1 || ((mbs * (int64_t)sizeof(T)) & 15) == 0);
+  int row_begin = 0;
+  if (vec) {
+    dim3 gv(nboxes, (ni + rpb - 1) / rpb);
+    hipLaunchKernelGGL(k_leaf_assemble_vec<T>, gv, dim3(256), 0, s, nboxes, p, rpb, D, Dsq, c11, c22, c1, c2, c, M,
+                       ldm, mbs, xnodes, nxc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nrows == ni) return e;
+    row_begin = ni;
+  }
+  dim3 grid(nboxes, (nrows - row_begin + rpb - 1) / rpb);
+  hipLaunchKernelGGL(k_leaf_assemble<T>, grid, dim3(256), 0, s, nboxes, p, row_begin, nrows, rpb, D, Dsq, c11, c22,
+                     c12, c1, c2, c, Nxtot, M, ldm, mbs, xnodes, nxc);
   return hipGetLastError();
 }
 
