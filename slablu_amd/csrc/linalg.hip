@@ -460,11 +460,211 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int batch, int n, int d0, int 
   }
 }
 
+// X <- U^{-1} X for large FP32 batches with U upper (non-unit diagonal,
+// m x m, m % 4 == 0) and X m x n (n <= 16 NCT, n % 4 == 0, rows of both
+// 16-byte aligned): one workgroup per matrix
+// with all of X in LDS, 32-row blocks from the bottom, left-looking:
+//   Y_k = X_k - U[k, k+1:] X[k+1:]   (MFMA, U read from HBM once),
+//   X_k = U_kk^{-1} Y_k              (substitution, one thread per column).
+// The K index of each 16 x 16 x 4 MFMA step is permuted so that a lane's
+// four U values of a 16-wide K chunk are one 16-byte load: at step s, lane
+// l pairs U[i][k0 + 4 (l / 16) + s] with X[k0 + 4 (l / 16) + s][j].  Wave w
+// owns row tile w & 1 of the block and column tiles w / 2, w / 2 + 2, ...
+// Rows past m are zero padding and stay zero (the padded diagonal block
+// is the identity).  A lane's U values of a whole block are loaded at once,
+// and the next block's during this block's substitution.
+constexpr int kTrsmBlk = 32;
+constexpr size_t kTrsmLdsMax = 160 * 1024;  // LDS per workgroup on gfx950
+constexpr int kTrsmMaxCh = 26;              // 16-wide K chunks of a block's product: m <= 448
+
+__host__ __device__ inline int trsm_up_ldx(int ntile) { return 16 * ntile; }
+__host__ __device__ inline size_t trsm_up_lds(int m, int ntile) {
+  const int mp = ((m + kTrsmBlk - 1) / kTrsmBlk) * kTrsmBlk;
+  return sizeof(float) * ((size_t)mp * trsm_up_ldx(ntile) + kTrsmBlk * (kTrsmBlk + 1));
+}
+
+template <int NCT>
+__global__ __launch_bounds__(256) void k_trsm_upper_mfma(int batch, int m, int n, View<float> A, View<float> B) {
+  extern __shared__ float lds_f[];
+  constexpr int ldx = 16 * NCT;
+  constexpr int CPW = (NCT + 1) / 2;  // column tiles per wavefront, at most
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef Mfma<float>::acc_t acc_t;
+  const int mp = ((m + kTrsmBlk - 1) / kTrsmBlk) * kTrsmBlk;
+  float* Xs = lds_f;
+  float* Us = Xs + mp * ldx;  // kTrsmBlk x (kTrsmBlk + 1)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int rt = wave & 1, ct0 = wave >> 1;
+  const int mm1 = m - 1;
+  // this lane's U values of a block's product, all K chunks at once: rows
+  // R0 + 16 rt + (l % 16), columns k0s + 16 ch + 4 (l / 16) .. + 3 (chunks
+  // past the last one read a clamped address and are never used)
+  v4 a[kTrsmMaxCh];
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    // the block's diagonal block: thread t holds row t / 8, columns
+    // 4 (t % 8) .. + 3 (clamped; masked when stored)
+    v4 ud;
+    auto fetch = [&](int R0) __attribute__((always_inline)) {
+      const int row = R0 + rt * 16 + i16;
+      const float* arow = A.at(b, row < mm1 ? row : mm1, 0);
+      const int k0s = R0 + kTrsmBlk;
+#pragma unroll
+      for (int ch = 0; ch < kTrsmMaxCh; ++ch) {
+        const int kb = k0s + ch * 16 + 4 * kq;
+        a[ch] = *reinterpret_cast<const v4*>(arow + (kb < m ? kb : m - 4));
+      }
+      const int dr = R0 + (tid >> 3), dc = R0 + 4 * (tid & 7);
+      ud = *reinterpret_cast<const v4*>(A.at(b, dr < mm1 ? dr : mm1, dc < m ? dc : m - 4));
+    };
+    fetch(mp - kTrsmBlk);
+    // X into LDS as 16-byte vectors, 8 loads in flight per thread (columns
+    // past n and rows past m as zeros)
+    {
+      constexpr int Q = ldx / 4;
+      const int nq = mp * Q;
+      const int nv4 = (n + 3) / 4;
+      for (int base = 0; base < nq; base += 8 * 256) {
+        v4 t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 256 + tid;
+          const int r = e / Q, c4 = e % Q;
+          const int rc = r < mm1 ? r : mm1, cc = c4 < nv4 - 1 ? c4 : nv4 - 1;
+          t[u] = *reinterpret_cast<const v4*>(B.at(b, rc, 4 * cc));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 256 + tid;
+          const int r = e / Q, c4 = e % Q;
+          if (e < nq) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Xs[r * ldx + 4 * c4 + q] = ((r < m) & (4 * c4 + q < n)) ? t[u][q] : 0.f;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int R0 = mp - kTrsmBlk; R0 >= 0; R0 -= kTrsmBlk) {
+      // the diagonal block (fetched with the block's U values), identity
+      // past m
+      {
+        const int r = tid >> 3, c0 = 4 * (tid & 7);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = R0 + r, cc = R0 + c0 + q;
+          Us[r * (kTrsmBlk + 1) + c0 + q] = ((rr < m) & (cc < m)) ? ud[q] : ((r == c0 + q) ? 1.f : 0.f);
+        }
+      }
+      const int k0s = R0 + kTrsmBlk;
+      if (k0s < m) {
+        acc_t acc[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) acc[j] = acc_t{0.f, 0.f, 0.f, 0.f};
+        const int nch = (m - k0s + 15) / 16;
+        int ctc[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) ctc[j] = ct0 + 2 * j < NCT ? ct0 + 2 * j : NCT - 1;
+#pragma unroll
+        for (int ch = 0; ch < kTrsmMaxCh; ++ch) {
+          if (ch < nch) {
+            const int kb = k0s + ch * 16 + 4 * kq;
+            const bool kin = kb < m;
+            const int kr = kin ? kb : mm1;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+              const float av = kin ? a[ch][st] : 0.f;
+              const float* xr = Xs + (kr + (kin ? st : 0)) * ldx + i16;
+              // every slot runs (a slot past the last column tile reads a
+              // clamped tile and is not stored): no branch per MFMA
+#pragma unroll
+              for (int j = 0; j < CPW; ++j) acc[j] = Mfma<float>::mma(av, xr[ctc[j] * 16], acc[j]);
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+          const int ct = ct0 + 2 * j;
+          if (ct < NCT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int r = R0 + rt * 16 + Mfma<float>::row(lane, q);
+              if (r < m) Xs[r * ldx + ct * 16 + i16] -= acc[j][q];
+            }
+          }
+        }
+      }
+      // the next block's U values load during this block's substitution
+      if (R0 > 0) fetch(R0 - kTrsmBlk);
+      __syncthreads();
+      if (tid < n) {
+        float x[kTrsmBlk];
+#pragma unroll
+        for (int r = 0; r < kTrsmBlk; ++r) x[r] = Xs[(R0 + r) * ldx + tid];
+        // column-oriented: x_j is final once divided, then leaves every row
+        // above at once (the dependent chain is one division and one FMA per
+        // row, not a row's whole dot product)
+#pragma unroll
+        for (int j = kTrsmBlk - 1; j >= 0; --j) {
+          x[j] = x[j] / Us[j * (kTrsmBlk + 1) + j];
+#pragma unroll
+          for (int i = 0; i < j; ++i) x[i] -= Us[i * (kTrsmBlk + 1) + j] * x[j];
+        }
+#pragma unroll
+        for (int r = 0; r < kTrsmBlk; ++r) Xs[(R0 + r) * ldx + tid] = x[r];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < m * ldx; e += 256) {
+      const int r = e / ldx, c = e % ldx;
+      if (c < n) *B.at(b, r, c) = Xs[e];
+    }
+    __syncthreads();
+  }
+}
+
+// the fused upper solve when it applies (FP32, many matrices, X in LDS)
+template <typename T>
+static bool trsm_upper_fused(int batch, int m, int n, View<T> A, View<T> B, hipStream_t s, hipError_t* err) {
+  if constexpr (sizeof(T) != 4) {
+    return false;
+  } else {
+    const int nct = (n + 15) / 16;
+    if (batch < 64 || m % 4 != 0 || m < kTrsmBlk || m > kTrsmBlk + 16 * kTrsmMaxCh || n > 256 || A.cs != 1 || B.cs != 1 || (A.rs & 3) != 0 ||
+        (reinterpret_cast<uintptr_t>(A.p) & 15) != 0 || (batch > 1 && (A.bs & 3) != 0) || nct < 4 || nct > 6 ||
+        n % 4 != 0 || (B.rs & 3) != 0 || (reinterpret_cast<uintptr_t>(B.p) & 15) != 0 ||
+        (batch > 1 && (B.bs & 3) != 0) ||
+        trsm_up_lds(m, nct) > kTrsmLdsMax)
+      return false;
+    const size_t lds = trsm_up_lds(m, nct);
+    const int grid = batch < 65535 ? batch : 65535;
+#define SLAB_TRSM_UP(NCT_)                                                                                        \
+  if (nct == NCT_) {                                                                                              \
+    static bool attr = false;                                                                                     \
+    if (!attr) {                                                                                                  \
+      if ((*err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trsm_upper_mfma<NCT_>),                   \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrsmLdsMax)) != hipSuccess) \
+        return true;                                                                                              \
+      attr = true;                                                                                                \
+    }                                                                                                             \
+    hipLaunchKernelGGL(k_trsm_upper_mfma<NCT_>, dim3(grid), dim3(256), lds, s, batch, m, n, A, B);               \
+    *err = hipGetLastError();                                                                                     \
+    return true;                                                                                                  \
+  }
+    SLAB_TRSM_UP(4)
+    SLAB_TRSM_UP(5)
+    SLAB_TRSM_UP(6)
+#undef SLAB_TRSM_UP
+    return false;
+  }
+}
+
 template <typename T>
 hipError_t trsm_left(int batch, int m, int n, bool lower, bool unit, View<T> A, View<T> B, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
   dim3 grid(ceil_div(n, 64), grid_batch(batch));
   hipError_t e = hipSuccess;
+  if (!lower && !unit && trsm_upper_fused<T>(batch, m, n, A, B, s, &e)) return e;
   if (lower) {
     for (int d0 = 0; d0 < m; d0 += TB) {
       const int tb = (m - d0) < TB ? (m - d0) : TB;

@@ -375,15 +375,25 @@ def set_sweep_tile(dtype, bs, inv_factors, nr):
     return rc
 
 
+# most vectors one slab_mdot / slab_maxpy launch takes per column (their
+# per-thread partial sums are register arrays of this length); more run as
+# several launches
+KRYLOV_MAX = 32
+
+
 def mdot(V, w, out=None):
     """out[c, i] = V[c, i, :] . w[c, :] for all columns c at once (FP64):
     V (nc, k, n) and w (nc, n) with unit stride along n; out (nc, k)."""
     nc, k, n = V.shape
     _require(V.dtype == torch.float64 and w.dtype == torch.float64 and V.stride(2) == 1 and w.stride(-1) == 1 and
-             tuple(w.shape) == (nc, n) and 1 <= k <= 32, "mdot layouts")
+             tuple(w.shape) == (nc, n) and k >= 1, "mdot layouts")
     if out is None:
         out = torch.empty(nc, k, dtype=torch.float64, device=V.device)
     _require(out.stride(-1) == 1 and tuple(out.shape) == (nc, k), "mdot output layout")
+    if k > KRYLOV_MAX:
+        for c0 in range(0, k, KRYLOV_MAX):
+            mdot(V[:, c0:c0 + KRYLOV_MAX], w, out=out[:, c0:c0 + KRYLOV_MAX])
+        return out
     ws = torch.empty(max(1, int(LIB.slab_mdot_ws(nc, k, n)) // 8), dtype=torch.float64, device=V.device)
     _check(LIB.slab_mdot(nc, k, n, _p(V), V.stride(0), V.stride(1), _p(w), w.stride(0), _p(out), out.stride(0),
                          _p(ws), _stream()), "slab_mdot")
@@ -395,7 +405,11 @@ def maxpy(V, coef, w, alpha=1.0):
     nc, k, n = V.shape
     _require(V.dtype == torch.float64 and w.dtype == torch.float64 and coef.dtype == torch.float64 and
              V.stride(2) == 1 and w.stride(-1) == 1 and coef.stride(-1) == 1 and tuple(w.shape) == (nc, n) and
-             tuple(coef.shape) == (nc, k) and 1 <= k <= 32, "maxpy layouts")
+             tuple(coef.shape) == (nc, k) and k >= 1, "maxpy layouts")
+    if k > KRYLOV_MAX:
+        for c0 in range(0, k, KRYLOV_MAX):
+            maxpy(V[:, c0:c0 + KRYLOV_MAX], coef[:, c0:c0 + KRYLOV_MAX], w, alpha)
+        return w
     _check(LIB.slab_maxpy(nc, k, n, float(alpha), _p(V), V.stride(0), V.stride(1), _p(coef), coef.stride(0), _p(w),
                           w.stride(0), _stream()), "slab_maxpy")
     return w

@@ -140,6 +140,28 @@ def test_trsm(dtype, m, n, lower, unit, transA):
     assert _rel(X, ref) < 50 * TOL[dtype]
 
 
+@pytest.mark.parametrize("m,n", [(400, 80), (196, 56), (100, 96), (36, 52), (64, 80), (48, 49)])
+def test_trsm_upper_many_matrices(m, n):
+    """FP32 upper solves of 70 matrices (the leaf stage's S' = U^{-1} Y
+    shape: U and the right-hand sides side by side in one row-major
+    matrix), which run as one workgroup per matrix with the right-hand
+    sides in LDS: against a float64 triangular solve, for m not a multiple
+    of the 32-row block and n not a multiple of 16 (n = 49, not a multiple
+    of 4, takes the blocked path)."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(m + n)
+    b = 70
+    U0 = _tri(b, m, False, g, torch.float32)
+    Y0 = torch.randn(b, m, n, generator=g, dtype=torch.float64)
+    M = torch.cat((U0, Y0), dim=2).to("cuda", torch.float32).contiguous()
+    hip.trsm(M[:, :, :m], M[:, :, m:], lower=False, unit=False)
+    ref = torch.linalg.solve_triangular(M[:, :, :m].double().cpu(), Y0.float().double(), upper=True)
+    torch.cuda.synchronize()
+    assert _rel(M[:, :, m:].double().cpu(), ref) < 50 * TOL[torch.float32]
+    # the triangle itself is untouched
+    assert torch.equal(M[:, :, :m].cpu(), U0.float())
+
+
 def _lu_check(Af, LUf, piv, n):
     """||P L U - A|| / ||A|| for one square matrix (numpy)."""
     L = np.tril(LUf, -1) + np.eye(n)

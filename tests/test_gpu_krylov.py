@@ -7,7 +7,8 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("nc,k,n", [(1, 1, 5), (3, 5, 10007), (64, 12, 70001), (2, 32, 4096)])
+@pytest.mark.parametrize("nc,k,n", [(1, 1, 5), (3, 5, 10007), (64, 12, 70001), (2, 32, 4096), (2, 40, 4096),
+                                    (3, 70, 1001)])
 def test_mdot_maxpy(nc, k, n):
     from slablu_amd import hip
     g = torch.Generator().manual_seed(nc * 31 + k)
