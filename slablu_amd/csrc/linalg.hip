@@ -1119,11 +1119,13 @@ __global__ __launch_bounds__(NT) void k_getf2_reg(int batch, int nrows, int k0, 
 // up), and each column then moves its touched entries with independent
 // loads and stores, solving the unit-lower L11 (LDS) on the block rows in
 // registers.
-template <typename T, int W>
-__global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
+template <typename T, int W, int NT>
+__global__ __launch_bounds__(NT) void k_swap_trsm(int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
                                                   int64_t pbs) {
-  // the w block rows and up to w pivot rows get one slot per lane
+  // the w block rows and up to w pivot rows get one slot per lane of
+  // wavefront 0; every thread owns one column
   static_assert(W <= 32, "2 W slots must fit the 64 lanes");
+  static_assert(NT % 64 == 0 && NT >= W * W / 32, "swap-trsm shape");
   __shared__ T L[W][W + 1];
   __shared__ int s_piv[W];
   __shared__ int s_slot[W];
@@ -1131,38 +1133,50 @@ __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, 
   __shared__ int s_src[2 * W];
   __shared__ int s_cnt;
   const int tid = threadIdx.x;
-  const int col = blockIdx.x * 64 + tid;
+  const int col = blockIdx.x * NT + tid;
   const int cl = col < ncols ? col : ncols - 1;
   const int wm1 = w - 1;
   const bool right = col >= k0 + w;
   for (int b = blockIdx.y; b < batch; b += gridDim.y) {
-    if (tid < W) {
-      s_piv[tid] = (tid < w) ? ipiv[(int64_t)b * pbs + k0 + tid] - 1 : -1;
-      const int r = tid;
+    if (tid < W) s_piv[tid] = (tid < w) ? ipiv[(int64_t)b * pbs + k0 + tid] - 1 : -1;
+    if constexpr (NT == 64) {
+      // thread r < W loads row r of the block
+      if (tid < W) {
+        const int r = tid;
 #pragma unroll
-      for (int c = 0; c < W; ++c) {
+        for (int c = 0; c < W; ++c) {
+          const T v = *A.at(b, k0 + (r < wm1 ? r : wm1), k0 + (c < wm1 ? c : wm1));
+          L[r][c] = ((r < w) & (c < r)) ? v : T(0);
+        }
+      }
+    } else {
+      for (int e = tid; e < W * W; e += NT) {
+        const int r = e / W, c = e % W;
         const T v = *A.at(b, k0 + (r < wm1 ? r : wm1), k0 + (c < wm1 ? c : wm1));
         L[r][c] = ((r < w) & (c < r)) ? v : T(0);
       }
     }
     __syncthreads();
     const int i = tid;
-    const int p = (i < w) ? s_piv[i] : -1;
-    bool first = (i < w) && (p >= k0 + w);
-    int src_lane = i;
-    for (int q = 0; q < W; ++q) {
-      const bool hit = first & (q < i) & (s_piv[q] == p);
-      src_lane = (hit && src_lane == i) ? q : src_lane;
+    int p = -1, src_lane = i;
+    bool first = false;
+    if (tid < 64) {
+      p = (i < w) ? s_piv[i] : -1;
+      first = (i < w) && (p >= k0 + w);
+      for (int q = 0; q < W; ++q) {
+        const bool hit = first & (q < i) & (s_piv[q] == p);
+        src_lane = (hit && src_lane == i) ? q : src_lane;
+      }
+      first = first & (src_lane == i);
+      const unsigned long long mask = __ballot(first);
+      const int rank = __popcll(mask & ((1ull << i) - 1ull));
+      if (first) {
+        s_slot[i] = w + rank;
+        s_row[w + rank] = p;
+      }
+      if (i < w) s_row[i] = k0 + i;
+      if (i == 0) s_cnt = w + __popcll(mask);
     }
-    first = first & (src_lane == i);
-    const unsigned long long mask = __ballot(first);
-    const int rank = __popcll(mask & ((1ull << i) - 1ull));
-    if (first) {
-      s_slot[i] = w + rank;
-      s_row[w + rank] = p;
-    }
-    if (i < w) s_row[i] = k0 + i;
-    if (i == 0) s_cnt = w + __popcll(mask);
     __syncthreads();
     if (i < w) {
       if (p < k0 + w) s_slot[i] = p - k0;
@@ -1188,10 +1202,16 @@ __global__ __launch_bounds__(64) void k_swap_trsm(int batch, int ncols, int k0, 
       d[t] = *A.at(b, s_row[s_src[q]], cl);
     }
     if (right) {
+      // large batches: one column of L at a time (the compiler would
+      // otherwise load the whole block ahead into registers and halve the
+      // occupancy); the small-batch launches are latency-bound and keep the
+      // loads ahead
 #pragma unroll
-      for (int a = 0; a < W; ++a)
+      for (int a = 0; a < W; ++a) {
 #pragma unroll
         for (int r = a + 1; r < W; ++r) u[r] -= L[r][a] * u[a];
+        if constexpr (NT > 64) asm volatile("" ::: "memory");
+      }
     }
     if (col < ncols && (col < k0 || right)) {
 #pragma unroll
@@ -1261,13 +1281,21 @@ static hipError_t launch_panel(const Panel& P, int batch, int nrows, int ncols, 
 template <typename T>
 static hipError_t launch_swap_trsm(int W, int batch, int ncols, int k0, int w, View<T> A, const int* ipiv,
                                    int64_t pbs, hipStream_t s) {
+  // large batches (the leaf LUs): 256 columns per workgroup share the
+  // block and the slot bookkeeping; small ones (the interface): 64, for
+  // more workgroups per matrix
+  if (batch > 64 && W == 32) {
+    const dim3 g(ceil_div(ncols, 256), grid_batch(batch));
+    hipLaunchKernelGGL((k_swap_trsm<T, 32, 256>), g, dim3(256), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+    return hipGetLastError();
+  }
   const dim3 g(ceil_div(ncols, 64), grid_batch(batch));
   if (W == 32)
-    hipLaunchKernelGGL((k_swap_trsm<T, 32>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+    hipLaunchKernelGGL((k_swap_trsm<T, 32, 64>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
   else if (W == 16)
-    hipLaunchKernelGGL((k_swap_trsm<T, 16>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+    hipLaunchKernelGGL((k_swap_trsm<T, 16, 64>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
   else
-    hipLaunchKernelGGL((k_swap_trsm<T, 8>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
+    hipLaunchKernelGGL((k_swap_trsm<T, 8, 64>), g, dim3(64), 0, s, batch, ncols, k0, w, A, ipiv, pbs);
   return hipGetLastError();
 }
 
