@@ -376,9 +376,113 @@ hipError_t gemm_outer(int outer, int64_t ao, int64_t bo, int64_t co, int batch, 
   return e;
 }
 
+// C[b] = alpha A B[b] + beta C[b] for one A shared by every matrix (FP32,
+// m x k row-major with 16-byte rows, k % 4 == 0) and small outputs (m <=
+// 16 MT, n <= 16 NTL), as the leaf DtN product N_x S' (80 x 400 by 400 x
+// 80): one workgroup per matrix, the output's 16 x 16 tiles dealt
+// round-robin to the four wavefronts (a slot past the last tile repeats it
+// and is not stored), K in 16-wide chunks with the permuted K pairing of
+// k_trsm_upper_mfma (a lane's A values of a chunk are one 16-byte load);
+// the next chunk's operands load while this chunk's MFMAs run.
+template <int MT, int NTL>
+__global__ __launch_bounds__(256) void k_gemm_bcast_small(int batch, int m, int n, int k, float alpha, View<float> A,
+                                                          View<float> B, float beta, View<float> C) {
+  constexpr int NTILE = MT * NTL, TPW = (NTILE + 3) / 4;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef Mfma<float>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  int arow[TPW], bcol[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int t0 = wave + 4 * j, t = t0 < NTILE ? t0 : NTILE - 1;
+    const int r = (t / NTL) * 16 + i16, c = (t % NTL) * 16 + i16;
+    arow[j] = r < m ? r : m - 1;
+    bcol[j] = c < n ? c : n - 1;
+  }
+  for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+    acc_t acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] = acc_t{0.f, 0.f, 0.f, 0.f};
+    v4 an[TPW];
+    float bn[4][TPW];
+    auto fetch = [&](int k0) __attribute__((always_inline)) {
+      const int kb = k0 + 4 * kq;
+      const int kc = kb < k ? kb : k - 4;
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) an[j] = *reinterpret_cast<const v4*>(A.at(0, arow[j], kc));
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) bn[st][j] = *B.at(b, kc + st, bcol[j]);
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < k; k0 += 16) {
+      v4 av[TPW];
+      float bv[4][TPW];
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) {
+        av[j] = an[j];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) bv[st][j] = bn[st][j];
+      }
+      if (k0 + 16 < k) fetch(k0 + 16);
+      const bool kin = k0 + 4 * kq < k;
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) acc[j] = Mfma<float>::mma(kin ? av[j][st] : 0.f, bv[st][j], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      const int t = wave + 4 * j;
+      if (t < NTILE) {
+        const int c = (t % NTL) * 16 + i16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = (t / NTL) * 16 + Mfma<float>::row(lane, q);
+          if ((r < m) & (c < n)) {
+            float* cp = C.at(b, r, c);
+            *cp = beta != 0.f ? alpha * acc[j][q] + beta * *cp : alpha * acc[j][q];
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+static bool gemm_bcast_small(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C,
+                             hipStream_t s, hipError_t* err) {
+  if constexpr (sizeof(T) != 4) {
+    return false;
+  } else {
+    if (A.bs != 0 || batch < 64 || m < 49 || m > 80 || n < 49 || n > 80 || k < 64 || k % 4 != 0 || A.cs != 1 ||
+        (A.rs & 3) != 0 || (reinterpret_cast<uintptr_t>(A.p) & 15) != 0)
+      return false;
+    const int mt = (m + 15) / 16, nt = (n + 15) / 16;
+    const int grid = batch < 65535 ? batch : 65535;
+#define SLAB_GEMM_BS(MT_, NT_)                                                                                    \
+  if (mt == MT_ && nt == NT_) {                                                                                   \
+    hipLaunchKernelGGL((k_gemm_bcast_small<MT_, NT_>), dim3(grid), dim3(256), 0, s, batch, m, n, k, alpha, A, B,  \
+                       beta, C);                                                                                  \
+    *err = hipGetLastError();                                                                                     \
+    return true;                                                                                                  \
+  }
+    SLAB_GEMM_BS(4, 4)
+    SLAB_GEMM_BS(4, 5)
+    SLAB_GEMM_BS(5, 4)
+    SLAB_GEMM_BS(5, 5)
+#undef SLAB_GEMM_BS
+    return false;
+  }
+}
+
 template <typename T>
 hipError_t gemm(int batch, int m, int n, int k, T alpha, View<T> A, View<T> B, T beta, View<T> C, hipStream_t s) {
   if (batch <= 0 || m <= 0 || n <= 0) return hipSuccess;
+  hipError_t bs_err;
+  if (gemm_bcast_small<T>(batch, m, n, k, alpha, A, B, beta, C, s, &bs_err)) return bs_err;
   if (n <= 8 && A.cs == 1 && k >= 32 && k <= 256) {
     const int gx = (m + 15) / 16 < 8192 ? (m + 15) / 16 : 8192;
     dim3 grid(gx, grid_batch(batch));

@@ -108,6 +108,24 @@ def test_gemm_broadcast_and_zero_beta():
     assert _rel(C, A0 @ B0) < 1e-13
 
 
+@pytest.mark.parametrize("m,n,k,beta", [(80, 80, 400, 1.0), (56, 56, 196, 1.0), (49, 80, 64, 0.0), (80, 64, 100, 0.5)])
+def test_gemm_shared_operator_small_outputs(m, n, k, beta):
+    """FP32 products of one shared operator with 70 matrices into small
+    outputs (the leaf DtN product N_x S'), which run as one workgroup per
+    matrix: against the FP64 product; C is not read when beta is 0."""
+    hip = _hip()
+    g = torch.Generator().manual_seed(m + n + k)
+    A0 = torch.randn(m, k, generator=g, dtype=torch.float64)
+    B0 = torch.randn(70, k, n, generator=g, dtype=torch.float64)
+    C0 = torch.randn(70, m, n, generator=g, dtype=torch.float64)
+    C = C0.to("cuda", torch.float32) if beta != 0.0 else torch.full((70, m, n), float("nan"), device="cuda")
+    hip.gemm(A0.to("cuda", torch.float32), B0.to("cuda", torch.float32), C, alpha=-1.0, beta=beta)
+    ref = -(A0.float().double() @ B0.float().double()) + beta * C0.float().double()
+    torch.cuda.synchronize()
+    assert torch.isfinite(C).all()
+    assert _rel(C, ref) < TOL[torch.float32]
+
+
 def _tri(b, m, lower, g, dtype):
     A = torch.randn(b, m, m, generator=g, dtype=torch.float64) * 0.1
     A = torch.tril(A) if lower else torch.triu(A)
