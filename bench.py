@@ -436,8 +436,8 @@ def main():
         hbm = ai < ridge
         roof = dict(bound="hbm" if hbm else "mfma",
                     kernel="k_gemm<float, int, true, true>: FP32 MFMA GEMM (64 x 64 output tiles, 16-byte rows of "
-                           "A, B and C), every launch in the timed steps (leaf LU trailing updates and DtN "
-                           "products, slab Schur products)",
+                           "A, B and C), every launch in the timed steps (leaf LU trailing updates, slab Schur "
+                           "products; the leaf DtN product runs in its own small-output kernel)",
                     achieved=gbs if hbm else tflops, peak=PEAK_HBM_GBS if hbm else PEAK_TFLOPS["fp32"],
                     unit="GB/s" if hbm else "TFLOP/s",
                     frac=(gbs / PEAK_HBM_GBS) if hbm else (tflops / PEAK_TFLOPS["fp32"]),
