@@ -602,8 +602,9 @@ __global__ __launch_bounds__(256) void k_trsm_upper_mfma(int batch, int m, int n
   const int rt = wave & 1, ct0 = wave >> 1;
   const int mm1 = m - 1;
   // this lane's U values of a block's product, all K chunks at once: rows
-  // R0 + 16 rt + (l % 16), columns k0s + 16 ch + 4 (l / 16) .. + 3 (chunks
-  // past the last one read a clamped address and are never used)
+  // R0 + 16 rt + (l % 16), columns k0s + 16 ch + 4 (l / 16) .. + 3 (only the
+  // block's chunks are loaded; the last chunk's columns past m read a
+  // clamped address and are never used)
   v4 a[kTrsmMaxCh];
   for (int b = blockIdx.x; b < batch; b += gridDim.x) {
     // the block's diagonal block: thread t holds row t / 8, columns
@@ -613,10 +614,13 @@ __global__ __launch_bounds__(256) void k_trsm_upper_mfma(int batch, int m, int n
       const int row = R0 + rt * 16 + i16;
       const float* arow = A.at(b, row < mm1 ? row : mm1, 0);
       const int k0s = R0 + kTrsmBlk;
+      const int nchf = k0s < m ? (m - k0s + 15) / 16 : 0;  // same for every lane
 #pragma unroll
       for (int ch = 0; ch < kTrsmMaxCh; ++ch) {
-        const int kb = k0s + ch * 16 + 4 * kq;
-        a[ch] = *reinterpret_cast<const v4*>(arow + (kb < m ? kb : m - 4));
+        if (ch < nchf) {
+          const int kb = k0s + ch * 16 + 4 * kq;
+          a[ch] = *reinterpret_cast<const v4*>(arow + (kb < m ? kb : m - 4));
+        }
       }
       const int dr = R0 + (tid >> 3), dc = R0 + 4 * (tid & 7);
       ud = *reinterpret_cast<const v4*>(A.at(b, dr < mm1 ? dr : mm1, dc < m ? dc : m - 4));
